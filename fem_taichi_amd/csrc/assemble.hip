@@ -1,0 +1,429 @@
+// hex8 stiffness / mass assembly into the node-pair pattern, row-owner and
+// atomic-free.
+//
+// The reference (FEM_model_csr.py:150-165, 246-257; FEM_model_bsr.py:163-175)
+// computes an E x 8 x 8 x 3 x 3 element intermediate and scatters it with
+// atomic += after a linear slot search.  Here one work-group owns a group of
+// node rows (a spatially compact set, built once from the pattern):
+//   1. it stages, in LDS, the physical shape-function gradients
+//      a[e][k][g] = dN_k/dx at Gauss point g of every element touching its
+//      rows, and w[e][g] = E_e * detJ_g (one thread per (element, g));
+//   2. one thread per output block (row i, slot s) sums, over the row's
+//      incident elements in ascending element order, the 3x3 Gram block
+//      G = sum_g w_g a_gj (x) a_gk, where j, k are the local indices of the
+//      row node and the column node in that element;
+//   3. it writes K_block = lhat*G + mhat*G^T + mhat*tr(G)*I once.
+// Step 3 is the closed form of B_j^T D B_k with D = E*(lhat*1 1^T-block +
+// mhat*...) (lhat = nu/((1+nu)(1-2nu)), mhat = 1/(2(1+nu))): because K is
+// linear in G and E is folded into w, the combination is applied once per
+// block after the sum over elements.  The summation order is fixed, so the
+// result is bit-identical run to run; no float atomics are used.
+#include "common.hpp"
+
+namespace fem {
+namespace {
+
+constexpr int kAsmThreads = 256;
+
+struct AsmLds {
+  // byte offsets inside the dynamic LDS block
+  int a_off, w_off, wm_off, nodes_off, n_off, rowpre_off, rows_off, total;
+};
+
+template <typename T>
+__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bool want_m) {
+  AsmLds L;
+  int o = 0;
+  L.a_off = o;
+  if (want_k) o += eg * 8 * 8 * 3 * (int)sizeof(T);
+  o = (o + 15) & ~15;
+  L.w_off = o;
+  if (want_k) o += eg * 8 * (int)sizeof(T);
+  o = (o + 15) & ~15;
+  L.wm_off = o;
+  if (want_m) o += eg * 8 * (int)sizeof(double);
+  o = (o + 15) & ~15;
+  L.n_off = o;
+  o += 64 * (int)sizeof(double);
+  L.nodes_off = o;
+  o += eg * 8 * (int)sizeof(int32_t);
+  L.rowpre_off = o;
+  o += (rg + 1) * (int)sizeof(int32_t);
+  L.rows_off = o;
+  o += rg * (int)sizeof(int32_t);
+  L.total = (o + 15) & ~15;
+  return L;
+}
+
+template <typename T, bool WANT_K, bool WANT_M, int LAYOUT>
+__global__ void __launch_bounds__(kAsmThreads)
+hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar,
+                     double lhat, double mhat, double rho, T* __restrict__ k_vals,
+                     double* __restrict__ m_blocks, int eg, int rg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const AsmLds L = asm_lds_layout<T>(eg, rg, WANT_K, WANT_M);
+  T* s_a = reinterpret_cast<T*>(smem + L.a_off);
+  T* s_w = reinterpret_cast<T*>(smem + L.w_off);
+  double* s_wm = reinterpret_cast<double*>(smem + L.wm_off);
+  double* s_N = reinterpret_cast<double*>(smem + L.n_off);
+  int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + L.nodes_off);
+  int32_t* s_rowpre = reinterpret_cast<int32_t*>(smem + L.rowpre_off);
+  int32_t* s_rows = reinterpret_cast<int32_t*>(smem + L.rows_off);
+
+  const int grp = blockIdx.x;
+  const int e0 = m.grp_elem_ptr[grp];
+  const int ne = m.grp_elem_ptr[grp + 1] - e0;
+  const int r0 = m.grp_row_ptr[grp];
+  const int nr = m.grp_row_ptr[grp + 1] - r0;
+  const int tid = threadIdx.x;
+
+  // phase 0: element node lists, row list, shape table
+  for (int t = tid; t < ne * 8; t += kAsmThreads) {
+    int e = m.grp_elems[e0 + (t >> 3)];
+    s_nodes[t] = m.elem[(int64_t)e * 8 + (t & 7)];
+  }
+  for (int t = tid; t < nr; t += kAsmThreads) s_rows[t] = m.grp_rows[r0 + t];
+  if (WANT_M) {
+    if (tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);   // s_N[g*8 + k]
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int acc = 0;
+    for (int r = 0; r < nr; ++r) {
+      s_rowpre[r] = acc;
+      acc += m.co_node[s_rows[r]];
+    }
+    s_rowpre[nr] = acc;
+  }
+
+  // phase 1: geometry at each (element, Gauss point)
+  for (int t = tid; t < ne * 8; t += kAsmThreads) {
+    const int le = t >> 3, g = t & 7;
+    const int e = m.grp_elems[e0 + le];
+    double x0[3], J[9];
+    {
+      const int32_t n0 = s_nodes[le * 8];
+      for (int a = 0; a < 3; ++a) x0[a] = m.coords[(int64_t)n0 * 3 + a];
+    }
+    for (int q = 0; q < 9; ++q) J[q] = 0.0;
+    double dn[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double f[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) f[a] = 1.0 + node_sign(k, a) * gauss_sign(g, a) * kGS;
+      dn[k][0] = 0.125 * node_sign(k, 0) * f[1] * f[2];
+      dn[k][1] = 0.125 * node_sign(k, 1) * f[0] * f[2];
+      dn[k][2] = 0.125 * node_sign(k, 2) * f[0] * f[1];
+      const int32_t nk = s_nodes[le * 8 + k];
+      double x[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) x[a] = m.coords[(int64_t)nk * 3 + a] - x0[a];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(x[a], dn[k][d], J[a * 3 + d]);
+    }
+    if (WANT_K) {
+      double Ji[9];
+      const double det = inv3(J, Ji);
+      const double Ee = young ? young[e] : young_scalar;
+      s_w[le * 8 + g] = (T)(Ee * det);
+      if (WANT_M) s_wm[le * 8 + g] = rho * det;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          // DN[k][a] = sum_d DNr[k][d] * Jinv[d][a]
+          double v = dn[k][0] * Ji[0 * 3 + a] + dn[k][1] * Ji[1 * 3 + a] + dn[k][2] * Ji[2 * 3 + a];
+          s_a[((le * 8 + k) * 8 + g) * 3 + a] = (T)v;
+        }
+      }
+    } else if (WANT_M) {
+      s_wm[le * 8 + g] = rho * det3(J);
+    }
+  }
+  __syncthreads();
+
+  // phase 2: one thread per (row, slot) block, fixed element order
+  const int nslots = s_rowpre[nr];
+  for (int t = tid; t < nslots; t += kAsmThreads) {
+    int lo = 0, hi = nr - 1;   // last r with s_rowpre[r] <= t
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (s_rowpre[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    const int r = lo;
+    const int32_t i = s_rows[r];
+    const int mloc = t - s_rowpre[r];
+    const int32_t sm = m.sm_node[i];
+    const int64_t s = (int64_t)sm + mloc;
+    const int32_t c = m.re_node[s];
+    T G[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) G[q] = T(0);
+    double msum = 0.0;
+    const int32_t ib = m.ne_ptr[i], ie = m.ne_ptr[i + 1];
+    for (int32_t inc = ib; inc < ie; ++inc) {
+      const int j = m.ne_list[inc] & 7;
+      const int le = m.ne_local[inc];
+      const int32_t* nd = s_nodes + le * 8;
+#pragma unroll 1
+      for (int k = 0; k < 8; ++k) {
+        if (nd[k] != c) continue;
+        if (WANT_K) {
+          const T* aj = s_a + (le * 8 + j) * 24;
+          const T* ak = s_a + (le * 8 + k) * 24;
+          const T* w = s_w + le * 8;
+#pragma unroll
+          for (int g = 0; g < 8; ++g) {
+            const T wg = w[g];
+            const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
+            const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
+            G[0] = fma(u0, b0, G[0]); G[1] = fma(u0, b1, G[1]); G[2] = fma(u0, b2, G[2]);
+            G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
+            G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
+          }
+        }
+        if (WANT_M) {
+          const double* wm = s_wm + le * 8;
+#pragma unroll
+          for (int g = 0; g < 8; ++g) msum = fma(wm[g] * s_N[g * 8 + j], s_N[g * 8 + k], msum);
+        }
+      }
+    }
+    if (WANT_K) {
+      const T lh = (T)lhat, mh = (T)mhat;
+      const T tr = mh * (G[0] + G[4] + G[8]);
+      T Kb[9];
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          Kb[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
+      if (LAYOUT == FEM_LAYOUT_BSR) {
+        T* dst = k_vals + s * 9;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) dst[q] = Kb[q];
+      } else {
+        const int64_t co = m.co_node[i];
+        const int64_t base = 9 * (int64_t)sm + 3 * (int64_t)mloc;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) k_vals[base + 3 * p * co + q] = Kb[3 * p + q];
+      }
+    }
+    if (WANT_M) m_blocks[s] = msum;
+  }
+}
+
+template <typename T, bool K, bool M, int LAYOUT>
+int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, double rho,
+               void* k_vals, double* m_blocks, hipStream_t st) {
+  const int eg = m.max_group_elems, rg = m.max_group_rows;
+  const AsmLds L = asm_lds_layout<T>(eg, rg, K, M);
+  FEM_REQUIRE(L.total <= 160 * 1024, FEM_E_LIMIT,
+              "row group needs %d B of LDS (> 160 KB): use smaller row groups", L.total);
+  auto kern = hex8_assemble_kernel<T, K, M, LAYOUT>;
+  FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+  const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
+  const double mhat = 1.0 / (2 * (1 + nu));
+  hipLaunchKernelGGL(kern, dim3((unsigned)m.num_groups), dim3(kAsmThreads), L.total, st, m, young,
+                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR index arrays and value expansion (FEM_model_csr.py:163, 168-173, 189-201)
+// ---------------------------------------------------------------------------
+__global__ void csr_index_K_kernel(fem_mesh m, int32_t* __restrict__ row_k, int32_t* __restrict__ col_k) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < m.num_node) {
+    const int32_t co = m.co_node[s], sm = m.sm_node[s];
+    for (int p = 0; p < 3; ++p) row_k[3 * s + 1 + p] = co * 3 * (p + 1) + sm * 9;
+    if (s == 0) row_k[0] = 0;
+  }
+  if (s >= m.nnzb) return;
+  // row of slot s: last i with sm_node[i] <= s (binary search)
+  int64_t lo = 0, hi = m.num_node - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (m.sm_node[mid] <= s) lo = mid; else hi = mid - 1;
+  }
+  const int64_t i = lo;
+  const int64_t sm = m.sm_node[i], co = m.co_node[i], mloc = s - sm;
+  const int32_t c = m.re_node[s];
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) col_k[9 * sm + 3 * p * co + 3 * mloc + q] = 3 * c + q;
+}
+
+__global__ void csr_index_M_kernel(fem_mesh m, int32_t* __restrict__ row_m, int32_t* __restrict__ col_m,
+                                   const double* __restrict__ m_blocks, double* __restrict__ val_m) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row_m && s < m.num_node) {
+    const int32_t co = m.co_node[s], sm = m.sm_node[s];
+    for (int p = 0; p < 3; ++p) row_m[3 * s + 1 + p] = co * (p + 1) + sm * 3;
+    if (s == 0) row_m[0] = 0;
+  }
+  if (s >= m.nnzb) return;
+  int64_t lo = 0, hi = m.num_node - 1;
+  while (lo < hi) {
+    int64_t mid = (lo + hi + 1) >> 1;
+    if (m.sm_node[mid] <= s) lo = mid; else hi = mid - 1;
+  }
+  const int64_t i = lo;
+  const int64_t sm = m.sm_node[i], co = m.co_node[i], mloc = s - sm;
+  const int32_t c = m.re_node[s];
+  for (int p = 0; p < 3; ++p) {
+    const int64_t idx = 3 * sm + p * co + mloc;
+    if (col_m) col_m[idx] = 3 * c + p;
+    if (val_m) val_m[idx] = m_blocks[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Penalty (FEM_model_csr.py:233-242 / FEM_model_bsr.py:248-259): SET diagonal.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void penalty_kernel(fem_mesh m, const int32_t* __restrict__ bnode,
+                               const double* __restrict__ bflag, int64_t nb, T* __restrict__ vals,
+                               int layout, double value) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int32_t i = bnode[b];
+  if (i < 0 || i >= m.num_node) return;
+  const int32_t sm = m.sm_node[i], co = m.co_node[i];
+  int k = -1;
+  for (int q = 0; q < co; ++q)
+    if (m.re_node[sm + q] == i) { k = q; break; }
+  if (k < 0) return;
+  for (int j = 0; j < 3; ++j) {
+    if (bflag[3 * b + j] == -1.0) continue;
+    int64_t idx = (layout == FEM_LAYOUT_BSR) ? 9 * ((int64_t)sm + k) + 4 * j
+                                             : 9 * (int64_t)sm + 3 * (int64_t)j * co + 3 * k + j;
+    vals[idx] = (T)value;
+  }
+}
+
+int check_mesh(const fem_mesh* m, bool need_groups) {
+  FEM_REQUIRE(m != nullptr, FEM_E_INVALID, "mesh is NULL");
+  FEM_REQUIRE(m->num_node >= 0 && m->num_elem >= 0 && m->nnzb >= 0, FEM_E_INVALID, "negative size");
+  FEM_REQUIRE(m->nnzb * 9 < INT32_MAX, FEM_E_LIMIT, "9*nnzb must fit int32 (got nnzb=%lld)",
+              (long long)m->nnzb);
+  FEM_REQUIRE(m->sm_node && m->co_node && (m->nnzb == 0 || m->re_node), FEM_E_INVALID,
+              "pattern pointers are NULL");
+  if (need_groups) {
+    FEM_REQUIRE(m->coords && m->elem && m->ne_ptr && m->ne_list && m->grp_row_ptr && m->grp_rows &&
+                    m->grp_elem_ptr && m->grp_elems && m->ne_local,
+                FEM_E_INVALID, "mesh lacks coordinates, incidences or row groups");
+    FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_rows >= 0,
+                FEM_E_INVALID, "bad group sizes");
+  }
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int k_dtype, int want_k,
+                                               int want_m) {
+  if (k_dtype == FEM_DTYPE_F32) return asm_lds_layout<float>(max_group_elems, 64, want_k, want_m).total;
+  return asm_lds_layout<double>(max_group_elems, 64, want_k, want_m).total;
+}
+
+extern "C" int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_scalar,
+                                 double nu, double rho, void* k_vals, int k_dtype, int k_layout,
+                                 double* m_blocks, void* stream) {
+  int rc = check_mesh(mesh, true);
+  if (rc) return rc;
+  FEM_REQUIRE(k_vals || m_blocks, FEM_E_INVALID, "nothing to assemble (k_vals and m_blocks NULL)");
+  FEM_REQUIRE(k_dtype == FEM_DTYPE_F64 || k_dtype == FEM_DTYPE_F32, FEM_E_INVALID, "bad k_dtype");
+  FEM_REQUIRE(k_layout == FEM_LAYOUT_BSR || k_layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad k_layout");
+  FEM_REQUIRE(nu > -1.0 && nu < 0.5, FEM_E_INVALID, "Poisson ratio %g outside (-1, 0.5)", nu);
+  if (mesh->num_groups == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  const fem_mesh& m = *mesh;
+  const bool K = k_vals != nullptr, M = m_blocks != nullptr;
+  if (K && k_dtype == FEM_DTYPE_F64) {
+    if (k_layout == FEM_LAYOUT_BSR)
+      return M ? launch_asm<double, true, true, 0>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st)
+               : launch_asm<double, true, false, 0>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st);
+    return M ? launch_asm<double, true, true, 1>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st)
+             : launch_asm<double, true, false, 1>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st);
+  }
+  if (K) {
+    if (k_layout == FEM_LAYOUT_BSR)
+      return M ? launch_asm<float, true, true, 0>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st)
+               : launch_asm<float, true, false, 0>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st);
+    return M ? launch_asm<float, true, true, 1>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st)
+             : launch_asm<float, true, false, 1>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st);
+  }
+  return launch_asm<double, false, true, 0>(m, young, young_scalar, nu, rho, nullptr, m_blocks, st);
+}
+
+extern "C" int fem_csr_index_K(const fem_mesh* mesh, int32_t* row_k, int32_t* col_k, void* stream) {
+  int rc = check_mesh(mesh, false);
+  if (rc) return rc;
+  FEM_REQUIRE(row_k && (mesh->nnzb == 0 || col_k), FEM_E_INVALID, "null output");
+  int64_t n = mesh->nnzb > mesh->num_node ? mesh->nnzb : mesh->num_node;
+  if (n == 0) {
+    FEM_CHECK_HIP(hipMemsetAsync(row_k, 0, 4, as_stream(stream)));
+    return FEM_OK;
+  }
+  csr_index_K_kernel<<<ceil_div(n, 256), 256, 0, as_stream(stream)>>>(*mesh, row_k, col_k);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_csr_index_M(const fem_mesh* mesh, int32_t* row_m, int32_t* col_m, void* stream) {
+  int rc = check_mesh(mesh, false);
+  if (rc) return rc;
+  FEM_REQUIRE(row_m && (mesh->nnzb == 0 || col_m), FEM_E_INVALID, "null output");
+  int64_t n = mesh->nnzb > mesh->num_node ? mesh->nnzb : mesh->num_node;
+  if (n == 0) {
+    FEM_CHECK_HIP(hipMemsetAsync(row_m, 0, 4, as_stream(stream)));
+    return FEM_OK;
+  }
+  csr_index_M_kernel<<<ceil_div(n, 256), 256, 0, as_stream(stream)>>>(*mesh, row_m, col_m, nullptr,
+                                                                     nullptr);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_csr_values_M(const fem_mesh* mesh, const double* m_blocks, double* val_m,
+                                void* stream) {
+  int rc = check_mesh(mesh, false);
+  if (rc) return rc;
+  if (mesh->nnzb == 0) return FEM_OK;
+  FEM_REQUIRE(m_blocks && val_m, FEM_E_INVALID, "null pointer");
+  csr_index_M_kernel<<<ceil_div(mesh->nnzb, 256), 256, 0, as_stream(stream)>>>(
+      *mesh, nullptr, nullptr, m_blocks, val_m);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_apply_penalty(const fem_mesh* mesh, const int32_t* bound_node,
+                                 const double* bound_flag, int64_t num_bounds, void* k_vals,
+                                 int k_dtype, int k_layout, double value, void* stream) {
+  int rc = check_mesh(mesh, false);
+  if (rc) return rc;
+  if (num_bounds == 0) return FEM_OK;
+  FEM_REQUIRE(num_bounds > 0 && bound_node && bound_flag && k_vals, FEM_E_INVALID, "bad arguments");
+  FEM_REQUIRE(k_layout == FEM_LAYOUT_BSR || k_layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad layout");
+  hipStream_t st = as_stream(stream);
+  if (k_dtype == FEM_DTYPE_F64)
+    penalty_kernel<double><<<ceil_div(num_bounds, 256), 256, 0, st>>>(
+        *mesh, bound_node, bound_flag, num_bounds, reinterpret_cast<double*>(k_vals), k_layout, value);
+  else if (k_dtype == FEM_DTYPE_F32)
+    penalty_kernel<float><<<ceil_div(num_bounds, 256), 256, 0, st>>>(
+        *mesh, bound_node, bound_flag, num_bounds, reinterpret_cast<float*>(k_vals), k_layout, value);
+  else
+    FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype");
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}

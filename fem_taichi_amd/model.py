@@ -1,0 +1,174 @@
+"""Drop-in ``Model`` (the reference's FEM_model_csr.Model / FEM_model_bsr.Model
+linear part) backed by the fem355 HIP kernels.
+
+Public methods keep the reference's names and return types:
+``Get_MatrixKL()`` / ``Get_MatrixK()`` -> scipy CSR (FEM_model_csr.py:118-127),
+``Get_MatrixM()`` -> scipy CSR (:129-137), ``Get_VectorF()`` -> ndarray (3N,)
+(:141-146).  Device-resident forms for the GPU solvers are exposed as
+``K_bsr(dtype)``, ``M_blocks()`` and ``F_device()``.
+
+Deliberate deviations from the reference, each a fix of a reference quirk:
+repeated calls do not accumulate (the reference's fields are never reset,
+so a second ``Get_MatrixKL`` doubles K); nodal loads keep their float value
+unless ``int_loads=True`` (the reference casts them to int,
+FEM_model_csr.py:78); ``Materials['EX']`` may be a length-E array of
+per-element moduli (the reference has one scalar).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from . import _lib
+from .runtime import device, stream_ptr
+
+PENALTY = 1e20   # FEM_model_csr.py:241
+
+
+def stage_bounds(Bounds):
+    """FEM_model_csr.py:64-73: (4, nb) [ID-1, c0, c1, c2]; with no bounds the
+    reference stages one zero column, which fixes node Nind[0]."""
+    tmp = []
+    for ids, c in Bounds:
+        ids = np.asarray(ids, dtype=np.float64)[None, :]
+        cc = np.repeat(np.asarray(c, dtype=np.float64)[:, None], ids.shape[-1], axis=-1)
+        tmp.append(np.concatenate((ids, cc), axis=0))
+    return np.concatenate(tmp, axis=-1) if tmp else np.zeros((4, 1))
+
+
+def stage_forces(Forces, int_loads=False):
+    """FEM_model_csr.py:75-84 (with the int cast of :78 only when int_loads)."""
+    tmp = []
+    for ids, v in Forces:
+        ids = np.asarray(ids, dtype=np.float64)[None, :]
+        vv = np.asarray(v, dtype=np.float64)
+        if int_loads:
+            vv = np.trunc(vv)
+        tmp.append(np.concatenate((ids, np.repeat(vv[:, None], ids.shape[-1], axis=-1)), axis=0))
+    return np.concatenate(tmp, axis=-1) if tmp else np.zeros((4, 1))
+
+
+class Model:
+    """Linear hex8 model: stiffness K, consistent mass M and load F on the GPU."""
+
+    def __init__(self, name, Physics, Materials, Omegas=(0, 0, 100), Largedisp=False, int_loads=False):
+        self.name = name
+        self.DE, self.EX, self.PO = Materials['DE'], Materials['EX'], Materials['PO']
+        self.OM = np.array(Omegas, dtype=np.float64)
+        self.Physics = Physics
+        self.Largedisp = Largedisp
+        if getattr(Physics, "dev", None) is None:
+            Physics.pre_process()
+        self.dev = Physics.dev
+        self.Num_Node = Physics.Num_Node
+        self.Num_Elem = Physics.Num_Elem
+        self.Num_MaxN = Physics.Num_MaxN
+        self.Num_Free = self.Num_Node * 3
+        self.nnzb = int(Physics.SmNode[-1])
+        self.nnz_K, self.nnz_M = self.nnzb * 9, self.nnzb * 3
+        d = device()
+        ex = np.asarray(self.EX, dtype=np.float64)
+        if ex.ndim == 0:
+            self.young_scalar, self.young = float(ex), None
+        else:
+            if ex.shape != (self.Num_Elem,):
+                raise ValueError(f"Materials['EX'] must be a scalar or have shape ({self.Num_Elem},)")
+            self.young_scalar, self.young = 0.0, torch.as_tensor(ex).to(d)
+        bounds = stage_bounds(Physics.Bounds)
+        self.bounds_temp = bounds
+        nind = Physics.Nind
+        bid = bounds[0].astype(np.int64)
+        if bid.size and (bid.min() < 0 or bid.max() >= len(nind)):
+            raise ValueError("a bound node ID is outside the Node table")
+        self.bound_node = torch.as_tensor(nind[bid].astype(np.int32)).to(d)
+        self.bound_flag = torch.as_tensor(np.ascontiguousarray(bounds[1:4].T)).to(d)
+        forces = stage_forces(Physics.Forces, int_loads)
+        self.forces_temp = forces
+        load = np.zeros((self.Num_Node, 3))
+        fid = forces[0].astype(np.int64)
+        if fid.size and (fid.min() < 0 or fid.max() >= len(nind)):
+            raise ValueError("a loaded node ID is outside the Node table")
+        np.add.at(load, nind[fid], forces[1:4].T)
+        self.nodal_load = torch.as_tensor(load.reshape(-1)).to(d)
+        self._cache = {}
+
+    # ---------------------------------------------------------------- device forms
+    def _young_ptr(self):
+        return self.young.data_ptr() if self.young is not None else None
+
+    def assemble_K(self, dtype="f64", layout="bsr", penalty=True, out=None):
+        """K values (9*nnzb) on the device; BSR (nnzb,3,3) or the reference's CSR order."""
+        tdt = torch.float64 if dtype == "f64" else torch.float32
+        cdt = _lib.DTYPE_F64 if dtype == "f64" else _lib.DTYPE_F32
+        lay = _lib.LAYOUT_BSR if layout == "bsr" else _lib.LAYOUT_CSR
+        vals = out if out is not None else torch.empty(9 * self.nnzb, dtype=tdt, device=device())
+        st = stream_ptr()
+        _lib.call("fem_hex8_assemble", self.dev.ref, self._young_ptr(), self.young_scalar, float(self.PO),
+                  float(self.DE), vals.data_ptr(), cdt, lay, None, st)
+        if penalty:
+            nb = int(self.bound_node.numel())
+            _lib.call("fem_apply_penalty", self.dev.ref, self.bound_node.data_ptr(),
+                      self.bound_flag.data_ptr(), nb, vals.data_ptr(), cdt, lay, PENALTY, st)
+        return vals
+
+    def M_blocks(self):
+        """Mass coupling scalar of every node-pair block (nnzb,), device f64."""
+        if "M" not in self._cache:
+            mb = torch.empty(self.nnzb, dtype=torch.float64, device=device())
+            _lib.call("fem_hex8_assemble", self.dev.ref, None, 0.0, float(self.PO), float(self.DE), None,
+                      _lib.DTYPE_F64, _lib.LAYOUT_BSR, mb.data_ptr(), stream_ptr())
+            self._cache["M"] = mb
+        return self._cache["M"]
+
+    def K_bsr(self, dtype="f64"):
+        key = "K" + dtype
+        if key not in self._cache:
+            self._cache[key] = self.assemble_K(dtype, "bsr")
+        return self._cache[key]
+
+    def F_device(self):
+        """The load vector (3N,) f64 on the device."""
+        if "F" not in self._cache:
+            f = torch.empty(3 * self.Num_Node, dtype=torch.float64, device=device())
+            wb = _lib.lib().fem_body_force_work_bytes(self.Num_Elem)
+            work = torch.empty(wb, dtype=torch.uint8, device=device())
+            om = (ctypes.c_double * 3)(*[float(w) for w in self.OM])
+            _lib.call("fem_body_force", self.dev.ref, float(self.DE), om, self.nodal_load.data_ptr(),
+                      f.data_ptr(), work.data_ptr(), wb, stream_ptr())
+            self._cache["F"] = f
+        return self._cache["F"]
+
+    # ---------------------------------------------------------------- reference API
+    def Get_MatrixKL(self):
+        """FEM_model_csr.py:118-127: penalised stiffness as scipy CSR (3N x 3N)."""
+        vals = self.assemble_K("f64", "csr")
+        d = device()
+        row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
+        col = torch.empty(max(self.nnz_K, 1), dtype=torch.int32, device=d)
+        _lib.call("fem_csr_index_K", self.dev.ref, row.data_ptr(), col.data_ptr(), stream_ptr())
+        n = self.Num_Free
+        return sp.csr_matrix((vals.cpu().numpy(), col[:self.nnz_K].cpu().numpy(), row.cpu().numpy()),
+                             shape=(n, n), dtype=np.float64)
+
+    Get_MatrixK = Get_MatrixKL
+
+    def Get_MatrixM(self):
+        """FEM_model_csr.py:129-137: consistent mass as scipy CSR (3N x 3N)."""
+        mb = self.M_blocks()
+        d = device()
+        row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
+        col = torch.empty(max(self.nnz_M, 1), dtype=torch.int32, device=d)
+        val = torch.empty(max(self.nnz_M, 1), dtype=torch.float64, device=d)
+        st = stream_ptr()
+        _lib.call("fem_csr_index_M", self.dev.ref, row.data_ptr(), col.data_ptr(), st)
+        _lib.call("fem_csr_values_M", self.dev.ref, mb.data_ptr(), val.data_ptr(), st)
+        n = self.Num_Free
+        return sp.csr_matrix((val[:self.nnz_M].cpu().numpy(), col[:self.nnz_M].cpu().numpy(),
+                              row.cpu().numpy()), shape=(n, n), dtype=np.float64)
+
+    def Get_VectorF(self):
+        """FEM_model_csr.py:141-146: load vector in DOF order 3*i + p."""
+        return self.F_device().cpu().numpy()

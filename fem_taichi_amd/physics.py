@@ -1,0 +1,136 @@
+"""Mesh + pattern container with the reference's ``Read`` attributes.
+
+``Physics`` carries the duck type that the reference's ``Model`` and ``Post``
+consume (READ_physics.py:44-61, 96-97, 114, 134): ``Node`` (>=4, N) with
+1-based IDs in row 0 and x, y, z in the last three rows, ``Elem`` (>=8, E)
+whose last 8 rows are node ID - 1, ``Bounds``/``Forces`` lists,
+``Nind``, ``SmNode``, ``ReNode``, ``CoNode``, ``CoElem``, ``Num_Node``,
+``Num_Elem``, ``Num_MaxN``.  ``pre_process()`` builds the pattern on the GPU
+(fem_pattern_count / fem_pattern_fill) and keeps the device copies, plus the
+node -> element incidence list and the row groups, in ``self.dev``.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .groups import DEFAULT_BRICK, build_row_groups, node_group_ids
+from .runtime import device, stream_ptr
+
+
+class DeviceMesh:
+    """Device-resident mesh, pattern, incidences and row groups + the C descriptor."""
+
+    def __init__(self, coords, elem, pattern, groups):
+        self.t = {}
+        dev = device()
+        self.t["coords"] = torch.as_tensor(coords, dtype=torch.float64).contiguous().to(dev)
+        self.t["elem"] = torch.as_tensor(elem, dtype=torch.int32).contiguous().to(dev)
+        for k, v in pattern.items():
+            self.t[k] = v
+        for k in ("grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "ne_local"):
+            self.t[k] = torch.as_tensor(groups[k], dtype=torch.int32).to(dev)
+        self.num_node = int(coords.shape[0])
+        self.num_elem = int(elem.shape[0])
+        self.nnzb = int(pattern["re_node"].numel())
+        m = _lib.FemMesh()
+        m.num_node, m.num_elem, m.nnzb = self.num_node, self.num_elem, self.nnzb
+        for name in ("coords", "elem", "sm_node", "re_node", "co_node", "co_elem", "ne_ptr", "ne_list",
+                     "grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "ne_local"):
+            setattr(m, name, self.t[name].data_ptr() if self.t[name].numel() else None)
+        m.num_groups = int(groups["num_groups"])
+        m.max_group_elems = int(groups["max_group_elems"])
+        m.max_group_rows = int(groups["max_group_rows"])
+        self.c = m
+        self.groups_info = {k: groups[k] for k in ("num_groups", "max_group_elems", "max_group_rows")}
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.c)
+
+    def __getitem__(self, k):
+        return self.t[k]
+
+
+def gpu_pattern(elem_idx: torch.Tensor, N: int):
+    """Run the GPU pattern builder on (E, 8) int32 node indices (device)."""
+    dev = device()
+    E = int(elem_idx.shape[0])
+    st = stream_ptr()
+    co_elem = torch.empty(N, dtype=torch.int32, device=dev)
+    ne_ptr = torch.empty(N + 1, dtype=torch.int32, device=dev)
+    ne_list = torch.empty(max(8 * E, 1), dtype=torch.int32, device=dev)
+    co_node = torch.empty(N, dtype=torch.int32, device=dev)
+    sm_node = torch.empty(N + 1, dtype=torch.int32, device=dev)
+    wb = _lib.lib().fem_pattern_work_bytes(E, N)
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    _lib.call("fem_pattern_count", elem_idx.data_ptr() if E else None, E, N, co_elem.data_ptr(),
+              ne_ptr.data_ptr(), ne_list.data_ptr(), co_node.data_ptr(), sm_node.data_ptr(),
+              work.data_ptr(), wb, st)
+    nnzb = int(sm_node[N].item())
+    re_node = torch.empty(max(nnzb, 1), dtype=torch.int32, device=dev)
+    _lib.call("fem_pattern_fill", elem_idx.data_ptr() if E else None, N, ne_ptr.data_ptr(),
+              ne_list.data_ptr(), sm_node.data_ptr(), re_node.data_ptr(), st)
+    return dict(co_elem=co_elem, ne_ptr=ne_ptr, ne_list=ne_list[:8 * E], co_node=co_node,
+                sm_node=sm_node, re_node=re_node[:nnzb])
+
+
+class Physics:
+    """A mesh with boundary sets, in the reference's ``Read`` layout."""
+
+    def __init__(self, Node, Elem, Bounds=None, Forces=None, name="mesh", node_dims=None,
+                 brick=DEFAULT_BRICK, run_pre_process=True):
+        self.name = name
+        self.Node = np.asarray(Node)
+        self.Elem = np.asarray(Elem)
+        self.Bounds = list(Bounds) if Bounds is not None else []
+        self.Forces = list(Forces) if Forces is not None else []
+        self.node_dims = node_dims
+        self.brick = brick
+        self.dev = None
+        if run_pre_process:
+            self.pre_process()
+
+    @classmethod
+    def from_grid(cls, grid, **kw):
+        """From a ``mesh.StructuredGrid`` (keeps its brick-able node layout)."""
+        return cls(grid.Node, grid.Elem, grid.Bounds, grid.Forces, name=grid.name,
+                   node_dims=grid.node_dims, **kw)
+
+    def pre_process(self):
+        """READ_physics.py:42-61 on the GPU; the outputs equal SciPy's bit for bit."""
+        self.Num_Node = self.Node.shape[-1]
+        self.Num_Elem = self.Elem.shape[-1]
+        ids = self.Node[0, :].astype(np.int64)
+        self.Num_MaxN = int(ids.max()) if self.Num_Node else 0
+        if self.Num_Node and ids.min() < 1:
+            raise ValueError("node IDs must be >= 1")
+        self.Nind = np.zeros((self.Num_MaxN,), dtype=np.int64)
+        self.Nind[ids - 1] = np.arange(self.Num_Node, dtype=np.int64)
+        elem_ids = np.asarray(self.Elem[-8:], dtype=np.int64)
+        if elem_ids.size and (elem_ids.min() < 0 or elem_ids.max() >= self.Num_MaxN):
+            raise ValueError("element references a node ID outside the Node table")
+        en = self.Nind[elem_ids].T.astype(np.int32)          # (E, 8) node indices
+        self.elem_nodes = np.ascontiguousarray(en)
+        dev = device()
+        elem_t = torch.as_tensor(self.elem_nodes).to(dev)
+        pat = gpu_pattern(elem_t, self.Num_Node)
+        self.SmNode = pat["sm_node"].cpu().numpy()
+        self.ReNode = pat["re_node"].cpu().numpy()
+        self.CoNode = pat["co_node"].cpu().numpy()
+        self.CoElem = pat["co_elem"].cpu().numpy()
+        ne_ptr = pat["ne_ptr"].cpu().numpy()
+        ne_list = pat["ne_list"].cpu().numpy()
+        coords = np.ascontiguousarray(np.asarray(self.Node[-3:], dtype=np.float64).T)
+        gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=self.brick)
+        lds_cap = 160 * 1024
+        max_elems = 48
+        while (max_elems < 4096 and
+               _lib.lib().fem_hex8_assemble_lds_bytes(max_elems * 2, _lib.DTYPE_F64, 1, 1) <= lds_cap // 2):
+            max_elems *= 2
+        groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, max_elems)
+        self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
+        return self

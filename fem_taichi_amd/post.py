@@ -1,0 +1,46 @@
+"""Drop-in ``Post`` (the reference's POST_process.Post stress part) on the GPU.
+
+``Get_Stress()`` returns the (N, 10) array of POST_process.py:72-99:
+[sxx, syy, szz, txy, tyz, txz, principal 1..3 (descending), von Mises term]
+computed by fem_stress_recover (nodal averaging over CoElem, the
+reference's Gauss-to-node extrapolation map as written, Jacobi eigenvalues
+in FP64 and the reference's von Mises expression).  The Tecplot writers of
+the reference are outside this package.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .runtime import device, stream_ptr
+
+
+class Post:
+    def __init__(self, name, Physics, Materials, Solutions):
+        self.name = name
+        self.DE, self.EX, self.PO = Materials['DE'], Materials['EX'], Materials['PO']
+        self.Physics = Physics
+        if getattr(Physics, "dev", None) is None:
+            Physics.pre_process()
+        self.Num_Node = Physics.Node.shape[-1]
+        self.Num_Elem = Physics.Elem.shape[-1]
+        if isinstance(Solutions, torch.Tensor):
+            sol = Solutions.detach().double().cpu().numpy()
+        else:
+            sol = np.asarray(Solutions, dtype=np.float64)
+        # the reference keeps Solutions.squeeze().T, i.e. (3, N)   (POST_process.py:28)
+        self.Solutions = sol.squeeze().T if sol.ndim > 1 else sol.reshape(-1, 3).T
+        self._u = torch.as_tensor(np.ascontiguousarray(self.Solutions.T).reshape(-1)).to(device())
+
+    def Get_Stress(self):
+        N, E = self.Num_Node, self.Num_Elem
+        ex = np.asarray(self.EX, dtype=np.float64)
+        young = None if ex.ndim == 0 else torch.as_tensor(ex).to(device())
+        out = torch.empty(N * 10, dtype=torch.float64, device=device())
+        wb = _lib.lib().fem_stress_work_bytes(E)
+        work = torch.empty(wb, dtype=torch.uint8, device=device())
+        _lib.call("fem_stress_recover", self.Physics.dev.ref, young.data_ptr() if young is not None else None,
+                  float(ex) if ex.ndim == 0 else 0.0, float(self.PO), self._u.data_ptr(), out.data_ptr(),
+                  work.data_ptr(), wb, stream_ptr())
+        return out.cpu().numpy().reshape(N, 10)

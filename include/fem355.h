@@ -1,0 +1,203 @@
+/*
+ * fem355 -- C ABI of the MI355X (gfx950) hex8 FEM hot path.
+ *
+ * Every entry point takes DEVICE pointers (e.g. torch.Tensor.data_ptr()),
+ * sizes as int64_t and a hipStream_t passed as void*.  Calls are enqueued on
+ * that stream and return immediately unless stated otherwise.  Every entry
+ * point returns an int status: FEM_OK (0) or a negative FEM_E_* code, with a
+ * message available from fem_last_error() (thread-local).  No C++ exception
+ * crosses this boundary.  The library never frees caller memory and never
+ * allocates device memory inside an entry point: scratch space is passed in by
+ * the caller, sized by the matching *_work_bytes() query.
+ *
+ * Index conventions (the reference's, READ_physics.py:42-61):
+ *   node index i in [0, N), element e in [0, E);
+ *   elem[e*8 + j]  = node index of local node j (ANSYS I..P order);
+ *   sm_node (N+1)  = SmNode, block-row pointer of the node-pair pattern;
+ *   re_node (nnzb) = ReNode, sorted column node indices per row;
+ *   co_node (N)    = CoNode, co_elem (N) = CoElem;
+ *   ne_ptr (N+1), ne_list: node -> element incidences, entry = e*8 + j,
+ *                  sorted by (e, j) within each node (all (e, j) pairs,
+ *                  including repeated nodes of a degenerate element).
+ * Vectors are node-interleaved: v[3*i + p].
+ * BSR values are (nnzb, 3, 3) row-major: vals[9*s + 3*p + q].
+ */
+#ifndef FEM355_H
+#define FEM355_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEM_OK 0
+#define FEM_E_INVALID (-1)   /* bad argument: null pointer, bad size, bad enum */
+#define FEM_E_HIP (-2)       /* HIP runtime error (message has the HIP string) */
+#define FEM_E_LIMIT (-3)     /* a mesh exceeds a documented kernel limit       */
+#define FEM_E_NUMERIC (-4)   /* singular pivot / non-finite value detected     */
+
+#define FEM_DTYPE_F64 0
+#define FEM_DTYPE_F32 1
+
+#define FEM_LAYOUT_BSR 0     /* vals[9*s + 3*p + q]                              */
+#define FEM_LAYOUT_CSR 1     /* the reference's CSR: 9*Sm[i] + 3*p*Co[i] + 3*m + q
+                                (FEM_model_csr.py:162)                          */
+
+/* Thread-local message describing the last failure. */
+const char* fem_last_error(void);
+/* ABI version, bumped on any signature change. */
+int fem_abi_version(void);
+/* Number of visible HIP devices (0 when none). */
+int fem_device_count(int* count);
+
+/* ------------------------------------------------------------------------
+ * Mesh/pattern descriptor: all pointers are device pointers.
+ * Row groups (grp_*) partition the node rows into spatially compact sets
+ * whose incident elements are staged together in LDS by the row-owner
+ * kernels; ne_local[t] is the index, within the group of ne_list[t]'s node,
+ * of element ne_list[t]/8 in that group's sorted element list.
+ * ---------------------------------------------------------------------- */
+typedef struct fem_mesh {
+  int64_t num_node;
+  int64_t num_elem;
+  int64_t nnzb;
+  const double* coords;        /* N x 3 */
+  const int32_t* elem;         /* E x 8 */
+  const int32_t* sm_node;      /* N + 1 */
+  const int32_t* re_node;      /* nnzb  */
+  const int32_t* co_node;      /* N     */
+  const int32_t* co_elem;      /* N     */
+  const int32_t* ne_ptr;       /* N + 1 */
+  const int32_t* ne_list;      /* ne_ptr[N] */
+  int64_t num_groups;
+  int32_t max_group_elems;     /* max over groups of the element count */
+  int32_t max_group_rows;      /* max over groups of the row count     */
+  const int32_t* grp_row_ptr;  /* G + 1 */
+  const int32_t* grp_rows;     /* N, rows in group order */
+  const int32_t* grp_elem_ptr; /* G + 1 */
+  const int32_t* grp_elems;    /* sorted element ids per group */
+  const int32_t* ne_local;     /* ne_ptr[N] */
+} fem_mesh;
+
+/* ---- a1: symbolic pattern (replaces READ_physics.py:42-61 pre_process) --- */
+
+/* Scratch bytes needed by fem_pattern_count. */
+int64_t fem_pattern_work_bytes(int64_t num_elem, int64_t num_node);
+
+/* Builds CoElem, the node->element incidence list (sorted) and, per node, the
+ * neighbour count CoNode and its exclusive scan SmNode.  Synchronises the
+ * stream (the caller needs nnzb = sm_node[N] to size re_node).
+ * ne_list must hold 8*E entries.  FEM_E_INVALID if an element entry is
+ * outside [0, N) (checked before any scatter); FEM_E_LIMIT if a node has
+ * more than 128 distinct neighbour nodes (the per-thread sort buffer). */
+int fem_pattern_count(const int32_t* elem, int64_t num_elem, int64_t num_node,
+                      int32_t* co_elem, int32_t* ne_ptr, int32_t* ne_list,
+                      int32_t* co_node, int32_t* sm_node,
+                      void* work, int64_t work_bytes, void* stream);
+
+/* Writes ReNode: sorted unique neighbour node indices per row. */
+int fem_pattern_fill(const int32_t* elem, int64_t num_node, const int32_t* ne_ptr,
+                     const int32_t* ne_list, const int32_t* sm_node, int32_t* re_node,
+                     void* stream);
+
+/* ---- a2/a3: CSR index arrays of the reference (FEM_model_csr.py:163,
+ *      168-173, 191, 196-201) ---------------------------------------------- */
+
+/* RowK (3N+1) and ColK (9*nnzb) of InstallIndexK / InstallMatrixK. */
+int fem_csr_index_K(const fem_mesh* mesh, int32_t* row_k, int32_t* col_k, void* stream);
+/* RowM (3N+1) and ColM (3*nnzb) of InstallIndexM / InstallMatrixM. */
+int fem_csr_index_M(const fem_mesh* mesh, int32_t* row_m, int32_t* col_m, void* stream);
+/* ValM (3*nnzb) of the reference's CSR mass from the per-block scalars. */
+int fem_csr_values_M(const fem_mesh* mesh, const double* m_blocks, double* val_m, void* stream);
+
+/* ---- a4-a7: hex8 stiffness / mass assembly (replaces InstallMatrixK
+ *      FEM_model_csr.py:150-165, InstallMatrixKL FEM_model_bsr.py:163-175,
+ *      InstallMatrixM FEM_model_csr.py:178-192 with UnitMatrixK/M) ---------
+ * Row-owner, atomic-free: each node row's blocks are summed in a fixed
+ * element order and written exactly once.  young: per-element modulus (E)
+ * or NULL to use young_scalar.  k_vals (nullable): 9*nnzb values of dtype
+ * k_dtype in k_layout.  m_blocks (nullable): nnzb f64 scalars, the mass
+ * coupling of each node pair (the value the reference replicates on the
+ * three DOF diagonals).  Needs a mesh with row groups. */
+int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_scalar,
+                      double nu, double rho, void* k_vals, int k_dtype, int k_layout,
+                      double* m_blocks, void* stream);
+
+/* LDS bytes one work-group of fem_hex8_assemble uses for a group of
+ * max_group_elems elements (to check a grouping fits 160 KB before launch). */
+int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int k_dtype, int want_k, int want_m);
+
+/* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and
+ *      ModifyMatrixK FEM_model_bsr.py:248-259) ------------------------------
+ * For each bound entry b (node index bound_node[b], resolved by the host
+ * through Nind exactly as the reference does) and DOF j with
+ * bound_flag[3*b+j] != -1, SETS the node's diagonal value (j,j) to `value`
+ * (the reference uses 1e20).  Entries whose node is outside [0, N) or whose
+ * row has no diagonal block are skipped, as the reference's search skips
+ * them. */
+int fem_apply_penalty(const fem_mesh* mesh, const int32_t* bound_node, const double* bound_flag,
+                      int64_t num_bounds, void* k_vals, int k_dtype, int k_layout, double value,
+                      void* stream);
+
+/* ---- a9: load vector (replaces InstallVectorF FEM_model_csr.py:214-229 and
+ *      LoadF :205-210) -------------------------------------------------------
+ * f_out (3N) = centrifugal body force of every element gathered per node in
+ * element order + nodal_load (3N, nullable).  work: 24*E doubles. */
+int64_t fem_body_force_work_bytes(int64_t num_elem);
+int fem_body_force(const fem_mesh* mesh, double rho, const double* omega3,
+                   const double* nodal_load, double* f_out, void* work, int64_t work_bytes,
+                   void* stream);
+
+/* ---- a13: stress recovery (replaces get_NdSol/get_NdAve/UnitVectorS
+ *      POST_process.py:172-209 and the host eigh/von Mises :79-95) ---------
+ * u (3N) displacements; out (N x 10): [sxx syy szz txy tyz txz, p1 p2 p3
+ * (descending), the reference's von Mises term].  work: 48*E doubles. */
+int64_t fem_stress_work_bytes(int64_t num_elem);
+int fem_stress_recover(const fem_mesh* mesh, const double* young, double young_scalar,
+                       double nu, const double* u, double* out, void* work,
+                       int64_t work_bytes, void* stream);
+
+/* ---- SpMV / SpMM on the BSR(3x3) pattern (the K/M products inside
+ *      Lanczos_solve SOLVE_eigen.py:65-74 and the PCG forms) -------------- */
+
+/* y = A x for the 3x3-block matrix (sm_node, re_node, vals); dtype f64 or
+ * f32 (x, y of the same dtype). */
+int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, const void* x, void* y,
+                 void* stream);
+/* Y = A X for nrhs column-interleaved right-hand sides X[(3*i+p)*nrhs + c]. */
+int fem_bsr_spmm(const fem_mesh* mesh, const double* vals, const double* X, double* Y,
+                 int64_t nrhs, void* stream);
+/* y = M x with the scalar-per-block mass (the reference's CSR M). */
+int fem_mass_spmm(const fem_mesh* mesh, const double* m_blocks, const double* X, double* Y,
+                  int64_t nrhs, void* stream);
+
+/* ---- PCG building blocks (new GPU solver forms replacing the Pardiso call
+ *      SOLVE_static.py:31-38) ------------------------------------------------ */
+
+/* dinv (N x 9, dtype) = inverse of each node's 3x3 diagonal block.
+ * FEM_E_NUMERIC is reported through the flag (device int) when a block is
+ * singular or a row has no diagonal block. */
+int fem_bj_setup(const fem_mesh* mesh, const void* vals, int dtype, void* dinv,
+                 int32_t* flag, void* stream);
+
+/* Scratch bytes of the fused PCG kernels for n3 = 3N unknowns. */
+int64_t fem_pcg_work_bytes(int64_t num_node);
+
+/* The PCG scalar state lives in device memory (8 doubles): state[0]=rz,
+ * [1]=pq, [2]=alpha, [3]=beta, [4]=rr, [5]=bb, [6]=scratch. */
+/* r = b - A x; z = Dinv r; p = z; rz, rr, bb. */
+int fem_pcg_start(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
+                  const void* b, void* x, void* r, void* z, void* p, double* state,
+                  void* work, int64_t work_bytes, void* stream);
+/* One iteration: q = A p; alpha = rz / p.q; x += alpha p; r -= alpha q;
+ * z = Dinv r; beta = r.z / rz; p = z + beta p; rr = r.r. */
+int fem_pcg_iterate(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
+                    void* x, void* r, void* z, void* p, void* q, double* state,
+                    void* work, int64_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM355_H */
