@@ -1,0 +1,182 @@
+#!/usr/bin/env python
+"""Benchmark: hex8 stiffness assembly throughput (elements assembled per second).
+
+One step = one FP64 assembly of the penalised stiffness K of the whole mesh
+into the BSR(3x3) node-pair pattern (fem_hex8_assemble + fem_apply_penalty),
+inputs (coordinates, connectivity, per-element modulus, pattern, row groups)
+already resident in HBM.  The workload is the 60x60x55-element jittered slab
+with a lognormal per-element modulus (208,376 nodes, 198,000 elements,
+5,438,326 node-pair blocks).
+
+The JSON line also carries:
+  roofline      -- the assembly kernel's algorithmic bytes per launch
+                   (24 N + 32 E + 8 E + 72 nnzb) over its mean launch time
+                   (HIP events on the launch stream), against 8 TB/s;
+  cpu_baseline  -- the C/OpenMP restatement of the reference's CPU assembly
+                   (oracle/assemble_ref.c) on the same mesh, median of 5 after
+                   one warm-up run, on this host's cores;
+  f32           -- the same assembly into an FP32 BSR copy;
+  static_solve  -- (with --solve) seconds to an FP64-verified 1e-8 relative
+                   residual on the rotating twisted blade, per solver form.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--solve", action="store_true", help="also time the static solver forms")
+    ap.add_argument("--solve-mesh", default="400x50x50")
+    return ap.parse_args()
+
+
+def alg_bytes_K(N, E, nnzb, per_elem_E=True, val_bytes=8):
+    return 24 * N + 32 * E + (8 * E if per_elem_E else 0) + 9 * val_bytes * nnzb
+
+
+def load_traffic(key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get(key)
+
+
+def cpu_baseline(grid, phys):
+    import numpy as np
+    from oracle import c_oracle
+    threads = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    xyz = np.ascontiguousarray(grid.Node[1:].T)
+    en = phys.elem_nodes
+    ex = np.asarray(grid.EX)
+    args = (xyz, en, ex if ex.ndim else None, float(ex) if ex.ndim == 0 else 0.0, grid.PO,
+            phys.SmNode, phys.CoNode, phys.ReNode)
+    c_oracle.assemble_K_csr(*args, with_cols=False)          # warm-up (like excluding Taichi JIT)
+    times = []
+    for _ in range(5):
+        t = time.perf_counter()
+        c_oracle.assemble_K_csr(*args, with_cols=False)
+        times.append(time.perf_counter() - t)
+    med = statistics.median(times)
+    return {"value": grid.Num_Elem / med, "unit": "hex8 elems assembled/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
+            "kind": "port", "sample": f"full workload ({grid.Num_Elem} elements), FP64 CSR K, median of 5 runs "
+                                      f"({med:.3f} s each) after 1 warm-up"}
+
+
+def time_solves(mesh_spec):
+    import torch
+    from fem_taichi_amd import Model, Physics, mesh, solvers
+    nx, ny, nz = (int(v) for v in mesh_spec.split("x"))
+    g = mesh.twisted_blade(nx, ny, nz)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=g.Omegas)
+    F = model.F_device()
+    K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32"))
+    torch.cuda.synchronize()
+    out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "tol": 1e-8}
+    _, info = solvers.solve_pcg64(K64, F, tol=1e-8)
+    out["pcg_fp64_bj"] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual}
+    _, info = solvers.solve_mixed(K64, K32, F, tol=1e-8)
+    out["mixed_fp32_pcg_fp64_refine"] = {"seconds": info.seconds, "iterations": info.iterations,
+                                         "outer": info.outer, "rel_residual": info.rel_residual}
+    print(f"# solves {json.dumps(out)}", flush=True)
+    return out
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the assembly and solvers would shard).", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from fem_taichi_amd import Model, Physics, mesh
+
+    g = mesh.slab_60x60x55()
+    print(f"# workload slab_60x60x55: {g.Num_Node} nodes, {g.Num_Elem} elements", flush=True)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=g.Omegas)
+    N, E, nnzb = g.Num_Node, g.Num_Elem, model.nnzb
+    vals = torch.empty(9 * nnzb, dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        model.assemble_K("f64", "bsr", out=vals)
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        starts[s].record()
+        model.assemble_K("f64", "bsr", penalty=False, out=vals)
+        ends[s].record()
+        model.apply_penalty(vals, "f64", "bsr")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    kmean = statistics.mean(kern_ms)
+    ab = alg_bytes_K(N, E, nnzb, per_elem_E=True)
+    achieved = ab / (kmean * 1e-3) / 1e9
+    traffic = load_traffic("hex8_assemble_f64_bsr_slab_60x60x55")
+
+    # FP32 copy (same kernel, FP32 contraction/values)
+    v32 = torch.empty(9 * nnzb, dtype=torch.float32, device="cuda")
+    for _ in range(args.warmup):
+        model.assemble_K("f32", "bsr", out=v32)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        model.assemble_K("f32", "bsr", out=v32)
+    e1.record()
+    torch.cuda.synchronize()
+    f32_ms = e0.elapsed_time(e1) / args.steps
+
+    result = {
+        "metric": "hex8 elems assembled/s; static solve s @1e-8 resid (learned vs plain PCG)",
+        "value": E * args.steps / elapsed,
+        "unit": "hex8 elems assembled/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "slab_60x60x55 jitter 10% seed 1, lognormal E sigma 0.3, FP64 K into BSR(3x3)",
+                   "nodes": N, "elements": E, "nnzb": nnzb, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "hex8_assemble_kernel<double,K,BSR>", "kernel_ms": kmean,
+                     "alg_bytes_per_launch": ab},
+        "f32": {"ms_per_assembly": f32_ms, "elems_per_s": E / (f32_ms * 1e-3),
+                "alg_GBs": alg_bytes_K(N, E, nnzb, True, 4) / (f32_ms * 1e-3) / 1e9},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(g, phys)
+    if args.solve:
+        result["static_solve"] = time_solves(args.solve_mesh)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

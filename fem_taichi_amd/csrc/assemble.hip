@@ -4,54 +4,74 @@
 // The reference (FEM_model_csr.py:150-165, 246-257; FEM_model_bsr.py:163-175)
 // computes an E x 8 x 8 x 3 x 3 element intermediate and scatters it with
 // atomic += after a linear slot search.  Here one work-group owns a group of
-// node rows (a spatially compact set, built once from the pattern):
-//   1. it stages, in LDS, the physical shape-function gradients
-//      a[e][k][g] = dN_k/dx at Gauss point g of every element touching its
-//      rows, and w[e][g] = E_e * detJ_g (one thread per (element, g));
-//   2. one thread per output block (row i, slot s) sums, over the row's
-//      incident elements in ascending element order, the 3x3 Gram block
-//      G = sum_g w_g a_gj (x) a_gk, where j, k are the local indices of the
-//      row node and the column node in that element;
-//   3. it writes K_block = lhat*G + mhat*G^T + mhat*tr(G)*I once.
-// Step 3 is the closed form of B_j^T D B_k with D = E*(lhat*1 1^T-block +
-// mhat*...) (lhat = nu/((1+nu)(1-2nu)), mhat = 1/(2(1+nu))): because K is
-// linear in G and E is folded into w, the combination is applied once per
-// block after the sum over elements.  The summation order is fixed, so the
-// result is bit-identical run to run; no float atomics are used.
+// node rows (a spatially compact set built once from the pattern):
+//   phase 1: it stages in LDS the physical shape-function gradients
+//            a[e][k][g] = dN_k/dx at Gauss point g of every element touching
+//            its rows and w[e][g] = E_e * detJ_g (one thread per (e, g));
+//   phase 2: 16 lanes own one row: lane = (local column node k, half h of
+//            the Gauss points).  Round t processes the row's t-th incidence
+//            (element e, local row node j) in ascending element order: the
+//            lane forms the 3x3 Gram block sum_g w_g a_gj (x) a_gk over its
+//            four Gauss points, the two halves are combined by a lane
+//            shuffle, and the block is added to the row's LDS accumulator at
+//            the slot of column node k.  Within a round the 8 lanes of a row
+//            hit 8 distinct slots; rounds are ordered, so every block is a sum
+//            in ascending element order -- bit-identical run to run, no
+//            atomics;
+//   phase 3: every block of the group's rows is written to HBM once,
+//            coalesced, as K_block = lhat*G + mhat*G^T + mhat*tr(G)*I.
+// Phase 3's closed form is B_j^T D B_k with D = E*[lhat 1 1^T + 2 mhat I on
+// the normal block, mhat I on the shear block] (lhat = nu/((1+nu)(1-2nu)),
+// mhat = 1/(2(1+nu))): K is linear in G and E is folded into w, so the
+// combination is applied once per block after the sum over elements.
 #include "common.hpp"
 
 namespace fem {
 namespace {
 
 constexpr int kAsmThreads = 256;
+constexpr int kLanesPerRow = 16;
+constexpr int kRowsPerPass = kAsmThreads / kLanesPerRow;
+constexpr int kAStride = 25;   // (e, k) stride of a[], in elements of T: 24 used + 1 pad (LDS banks)
 
 struct AsmLds {
-  // byte offsets inside the dynamic LDS block
-  int a_off, w_off, wm_off, nodes_off, n_off, rowpre_off, rows_off, total;
+  int a_off, w_off, wm_off, n_off, nodes_off, dup_off, rows_off, rowpre_off, re_off, acc_off, accm_off, total;
 };
 
+__host__ __device__ inline int align16(int o) { return (o + 15) & ~15; }
+
 template <typename T>
-__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bool want_m) {
+__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, bool want_k, bool want_m) {
   AsmLds L;
   int o = 0;
   L.a_off = o;
-  if (want_k) o += eg * 8 * 8 * 3 * (int)sizeof(T);
-  o = (o + 15) & ~15;
+  if (want_k) o += eg * 8 * kAStride * (int)sizeof(T);
+  o = align16(o);
+  L.acc_off = o;
+  if (want_k) o += sg * 9 * (int)sizeof(T);
+  o = align16(o);
   L.w_off = o;
   if (want_k) o += eg * 8 * (int)sizeof(T);
-  o = (o + 15) & ~15;
+  o = align16(o);
   L.wm_off = o;
   if (want_m) o += eg * 8 * (int)sizeof(double);
-  o = (o + 15) & ~15;
+  o = align16(o);
+  L.accm_off = o;
+  if (want_m) o += sg * (int)sizeof(double);
+  o = align16(o);
   L.n_off = o;
-  o += 64 * (int)sizeof(double);
+  if (want_m) o += 64 * (int)sizeof(double);
   L.nodes_off = o;
   o += eg * 8 * (int)sizeof(int32_t);
-  L.rowpre_off = o;
-  o += (rg + 1) * (int)sizeof(int32_t);
+  L.dup_off = o;
+  o += eg * (int)sizeof(int32_t);
   L.rows_off = o;
   o += rg * (int)sizeof(int32_t);
-  L.total = (o + 15) & ~15;
+  L.rowpre_off = o;
+  o += (rg + 1) * (int)sizeof(int32_t);
+  L.re_off = o;
+  o += sg * (int)sizeof(int32_t);
+  L.total = align16(o);
   return L;
 }
 
@@ -59,16 +79,20 @@ template <typename T, bool WANT_K, bool WANT_M, int LAYOUT>
 __global__ void __launch_bounds__(kAsmThreads)
 hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar,
                      double lhat, double mhat, double rho, T* __restrict__ k_vals,
-                     double* __restrict__ m_blocks, int eg, int rg) {
+                     double* __restrict__ m_blocks, int eg, int rg, int sg) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const AsmLds L = asm_lds_layout<T>(eg, rg, WANT_K, WANT_M);
+  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, WANT_K, WANT_M);
   T* s_a = reinterpret_cast<T*>(smem + L.a_off);
+  T* s_acc = reinterpret_cast<T*>(smem + L.acc_off);
   T* s_w = reinterpret_cast<T*>(smem + L.w_off);
   double* s_wm = reinterpret_cast<double*>(smem + L.wm_off);
+  double* s_accm = reinterpret_cast<double*>(smem + L.accm_off);
   double* s_N = reinterpret_cast<double*>(smem + L.n_off);
   int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + L.nodes_off);
-  int32_t* s_rowpre = reinterpret_cast<int32_t*>(smem + L.rowpre_off);
+  int32_t* s_dup = reinterpret_cast<int32_t*>(smem + L.dup_off);
   int32_t* s_rows = reinterpret_cast<int32_t*>(smem + L.rows_off);
+  int32_t* s_rowpre = reinterpret_cast<int32_t*>(smem + L.rowpre_off);
+  int32_t* s_re = reinterpret_cast<int32_t*>(smem + L.re_off);
 
   const int grp = blockIdx.x;
   const int e0 = m.grp_elem_ptr[grp];
@@ -77,15 +101,13 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   const int nr = m.grp_row_ptr[grp + 1] - r0;
   const int tid = threadIdx.x;
 
-  // phase 0: element node lists, row list, shape table
+  // ---- phase 0: element node lists, rows, shape table
   for (int t = tid; t < ne * 8; t += kAsmThreads) {
-    int e = m.grp_elems[e0 + (t >> 3)];
+    const int e = m.grp_elems[e0 + (t >> 3)];
     s_nodes[t] = m.elem[(int64_t)e * 8 + (t & 7)];
   }
   for (int t = tid; t < nr; t += kAsmThreads) s_rows[t] = m.grp_rows[r0 + t];
-  if (WANT_M) {
-    if (tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);   // s_N[g*8 + k]
-  }
+  if (WANT_M && tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);   // s_N[g*8 + k]
   __syncthreads();
   if (tid == 0) {
     int acc = 0;
@@ -95,8 +117,27 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
     }
     s_rowpre[nr] = acc;
   }
+  for (int le = tid; le < ne; le += kAsmThreads) {
+    int dup = 0;
+    for (int a = 0; a < 8; ++a)
+      for (int b = a + 1; b < 8; ++b) dup |= (s_nodes[le * 8 + a] == s_nodes[le * 8 + b]);
+    s_dup[le] = dup;
+  }
+  __syncthreads();
+  const int nslots = s_rowpre[nr];
+  // row neighbour lists into LDS, accumulators to zero
+  for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
+    const int32_t i = s_rows[r];
+    const int32_t sm = m.sm_node[i];
+    const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
+    for (int q = tid % kLanesPerRow; q < co; q += kLanesPerRow) s_re[base + q] = m.re_node[sm + q];
+  }
+  if (WANT_K)
+    for (int t = tid; t < nslots * 9; t += kAsmThreads) s_acc[t] = T(0);
+  if (WANT_M)
+    for (int t = tid; t < nslots; t += kAsmThreads) s_accm[t] = 0.0;
 
-  // phase 1: geometry at each (element, Gauss point)
+  // ---- phase 1: geometry at each (element, Gauss point)
   for (int t = tid; t < ne * 8; t += kAsmThreads) {
     const int le = t >> 3, g = t & 7;
     const int e = m.grp_elems[e0 + le];
@@ -132,12 +173,10 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
       if (WANT_M) s_wm[le * 8 + g] = rho * det;
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
+        T* dst = s_a + (le * 8 + k) * kAStride + 3 * g;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          // DN[k][a] = sum_d DNr[k][d] * Jinv[d][a]
-          double v = dn[k][0] * Ji[0 * 3 + a] + dn[k][1] * Ji[1 * 3 + a] + dn[k][2] * Ji[2 * 3 + a];
-          s_a[((le * 8 + k) * 8 + g) * 3 + a] = (T)v;
-        }
+        for (int a = 0; a < 3; ++a)
+          dst[a] = (T)(dn[k][0] * Ji[0 * 3 + a] + dn[k][1] * Ji[1 * 3 + a] + dn[k][2] * Ji[2 * 3 + a]);
       }
     } else if (WANT_M) {
       s_wm[le * 8 + g] = rho * det3(J);
@@ -145,38 +184,35 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   }
   __syncthreads();
 
-  // phase 2: one thread per (row, slot) block, fixed element order
-  const int nslots = s_rowpre[nr];
-  for (int t = tid; t < nslots; t += kAsmThreads) {
-    int lo = 0, hi = nr - 1;   // last r with s_rowpre[r] <= t
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (s_rowpre[mid] <= t) lo = mid; else hi = mid - 1;
-    }
-    const int r = lo;
-    const int32_t i = s_rows[r];
-    const int mloc = t - s_rowpre[r];
-    const int32_t sm = m.sm_node[i];
-    const int64_t s = (int64_t)sm + mloc;
-    const int32_t c = m.re_node[s];
-    T G[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) G[q] = T(0);
-    double msum = 0.0;
-    const int32_t ib = m.ne_ptr[i], ie = m.ne_ptr[i + 1];
-    for (int32_t inc = ib; inc < ie; ++inc) {
-      const int j = m.ne_list[inc] & 7;
-      const int le = m.ne_local[inc];
-      const int32_t* nd = s_nodes + le * 8;
-#pragma unroll 1
-      for (int k = 0; k < 8; ++k) {
-        if (nd[k] != c) continue;
+  // ---- phase 2: incidence rounds, 16 lanes per row
+  {
+    const int lane = tid % kLanesPerRow;
+    const int k = lane >> 1, h = lane & 1;
+    for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
+      const int32_t i = s_rows[r];
+      const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
+      const int32_t ib = m.ne_ptr[i], ie = m.ne_ptr[i + 1];
+      for (int32_t inc = ib; inc < ie; ++inc) {
+        const int j = m.ne_list[inc] & 7;
+        const int le = m.ne_local[inc];
+        const int32_t c = s_nodes[le * 8 + k];
+        int lo = 0, hi = co - 1;   // slot of column node c in the row's sorted list
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (s_re[base + mid] < c) lo = mid + 1; else hi = mid;
+        }
+        const int slot = base + lo;
+        T G[9];
+        double ms = 0.0;
         if (WANT_K) {
-          const T* aj = s_a + (le * 8 + j) * 24;
-          const T* ak = s_a + (le * 8 + k) * 24;
+          const T* aj = s_a + (le * 8 + j) * kAStride;
+          const T* ak = s_a + (le * 8 + k) * kAStride;
           const T* w = s_w + le * 8;
 #pragma unroll
-          for (int g = 0; g < 8; ++g) {
+          for (int q = 0; q < 9; ++q) G[q] = T(0);
+#pragma unroll
+          for (int gg = 0; gg < 4; ++gg) {
+            const int g = 4 * h + gg;
             const T wg = w[g];
             const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
             const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
@@ -184,45 +220,84 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
             G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
             G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
           }
+#pragma unroll
+          for (int q = 0; q < 9; ++q) G[q] += __shfl_xor(G[q], 1, 64);
         }
         if (WANT_M) {
           const double* wm = s_wm + le * 8;
 #pragma unroll
-          for (int g = 0; g < 8; ++g) msum = fma(wm[g] * s_N[g * 8 + j], s_N[g * 8 + k], msum);
+          for (int gg = 0; gg < 4; ++gg) {
+            const int g = 4 * h + gg;
+            ms = fma(wm[g] * s_N[g * 8 + j], s_N[g * 8 + k], ms);
+          }
+          ms += __shfl_xor(ms, 1, 64);
+        }
+        if (h == 0) {
+          if (!s_dup[le]) {
+            if (WANT_K) {
+#pragma unroll
+              for (int q = 0; q < 9; ++q) s_acc[slot * 9 + q] += G[q];
+            }
+            if (WANT_M) s_accm[slot] += ms;
+          } else {
+            // a degenerate element repeats a node: add its columns one by one
+            for (int kk = 0; kk < 8; ++kk) {
+              if (kk == k) {
+                if (WANT_K)
+                  for (int q = 0; q < 9; ++q) s_acc[slot * 9 + q] += G[q];
+                if (WANT_M) s_accm[slot] += ms;
+              }
+              __builtin_amdgcn_wave_barrier();
+            }
+          }
         }
       }
     }
-    if (WANT_K) {
-      const T lh = (T)lhat, mh = (T)mhat;
-      const T tr = mh * (G[0] + G[4] + G[8]);
-      T Kb[9];
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int q = 0; q < 3; ++q)
-          Kb[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
+  }
+  __syncthreads();
+
+  // ---- phase 3: write every block of the group's rows once
+  if (WANT_K) {
+    const T lh = (T)lhat, mh = (T)mhat;
+    for (int t = tid; t < nslots * 9; t += kAsmThreads) {
+      const int q9 = t / 9, e9 = t - 9 * q9;
+      const int p = e9 / 3, q = e9 - 3 * p;
+      int lo = 0, hi = nr - 1;   // row of slot q9: last r with s_rowpre[r] <= q9
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rowpre[mid] <= q9) lo = mid; else hi = mid - 1;
+      }
+      const T* G = s_acc + q9 * 9;
+      T v = lh * G[3 * p + q] + mh * G[3 * q + p];
+      if (p == q) v += mh * (G[0] + G[4] + G[8]);
+      const int32_t i = s_rows[lo];
+      const int64_t sm = m.sm_node[i];
+      const int64_t mloc = q9 - s_rowpre[lo];
       if (LAYOUT == FEM_LAYOUT_BSR) {
-        T* dst = k_vals + s * 9;
-#pragma unroll
-        for (int q = 0; q < 9; ++q) dst[q] = Kb[q];
+        k_vals[9 * (sm + mloc) + e9] = v;
       } else {
-        const int64_t co = m.co_node[i];
-        const int64_t base = 9 * (int64_t)sm + 3 * (int64_t)mloc;
-#pragma unroll
-        for (int p = 0; p < 3; ++p)
-#pragma unroll
-          for (int q = 0; q < 3; ++q) k_vals[base + 3 * p * co + q] = Kb[3 * p + q];
+        const int64_t co = s_rowpre[lo + 1] - s_rowpre[lo];
+        k_vals[9 * sm + 3 * p * co + 3 * mloc + q] = v;
       }
     }
-    if (WANT_M) m_blocks[s] = msum;
+  }
+  if (WANT_M) {
+    for (int t = tid; t < nslots; t += kAsmThreads) {
+      int lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_rowpre[mid] <= t) lo = mid; else hi = mid - 1;
+      }
+      m_blocks[(int64_t)m.sm_node[s_rows[lo]] + (t - s_rowpre[lo])] = s_accm[t];
+    }
   }
 }
 
 template <typename T, bool K, bool M, int LAYOUT>
 int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, double rho,
                void* k_vals, double* m_blocks, hipStream_t st) {
-  const int eg = m.max_group_elems, rg = m.max_group_rows;
-  const AsmLds L = asm_lds_layout<T>(eg, rg, K, M);
+  const int eg = m.max_group_elems, rg = m.max_group_rows, sg = m.max_group_slots;
+  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, K, M);
   FEM_REQUIRE(L.total <= 160 * 1024, FEM_E_LIMIT,
               "row group needs %d B of LDS (> 160 KB): use smaller row groups", L.total);
   auto kern = hex8_assemble_kernel<T, K, M, LAYOUT>;
@@ -231,7 +306,7 @@ int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, dou
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
   hipLaunchKernelGGL(kern, dim3((unsigned)m.num_groups), dim3(kAsmThreads), L.total, st, m, young,
-                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg);
+                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg, sg);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -319,7 +394,8 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
     FEM_REQUIRE(m->coords && m->elem && m->ne_ptr && m->ne_list && m->grp_row_ptr && m->grp_rows &&
                     m->grp_elem_ptr && m->grp_elems && m->ne_local,
                 FEM_E_INVALID, "mesh lacks coordinates, incidences or row groups");
-    FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_rows >= 0,
+    FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_rows >= 0 &&
+                    m->max_group_slots >= 0,
                 FEM_E_INVALID, "bad group sizes");
   }
   return FEM_OK;
@@ -330,10 +406,12 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
 
 using namespace fem;
 
-extern "C" int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int k_dtype, int want_k,
+extern "C" int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
+                                               int32_t max_group_slots, int k_dtype, int want_k,
                                                int want_m) {
-  if (k_dtype == FEM_DTYPE_F32) return asm_lds_layout<float>(max_group_elems, 64, want_k, want_m).total;
-  return asm_lds_layout<double>(max_group_elems, 64, want_k, want_m).total;
+  if (k_dtype == FEM_DTYPE_F32)
+    return asm_lds_layout<float>(max_group_elems, max_group_rows, max_group_slots, want_k, want_m).total;
+  return asm_lds_layout<double>(max_group_elems, max_group_rows, max_group_slots, want_k, want_m).total;
 }
 
 extern "C" int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_scalar,

@@ -1,122 +1,95 @@
 // BSR(3x3) SpMV / SpMM, block-Jacobi, and the fused PCG iteration kernels.
 //
-// SpMV: a group of 16 lanes owns one block row.  The row's 9*CoNode values
-// are contiguous (BSR row-major blocks), so the 16 lanes stream them with
-// unit stride; lane l handles value v = l, l+16, ... (block v/9, entry
-// (v%9)/3, (v%9)%3) and the three row sums are reduced across the 16 lanes
-// with shuffles.  Dot products are reduced deterministically: each
-// work-group writes a partial, and the last work-group to finish (integer
-// atomic ticket) sums the partials in index order.
+// SpMV: a group of 32 lanes owns one block row; lane b loads block b's nine
+// values (the row's blocks are contiguous, so the group reads one contiguous
+// 72*CoNode-byte run), its column index and the three x entries of that
+// column, and the three row sums are reduced across the 32 lanes with
+// shuffles.  Grids are fixed (kGrid blocks, grid-stride over rows) so that
+// dot products are reduced without atomics: each of exactly kGrid blocks
+// writes one partial (zero when it had no rows),
+// and the kernel that needs the scalar re-sums the kGrid partials itself in
+// index order (every block gets the identical, deterministic value).
 #include "common.hpp"
 
 namespace fem {
 namespace {
 
-constexpr int kRowLanes = 16;
-constexpr int kSpmvThreads = 256;
-constexpr int kRowsPerBlock = kSpmvThreads / kRowLanes;   // 16
-constexpr int kVecThreads = 256;
+constexpr int kRowLanes = 32;
+constexpr int kThreads = 256;
+constexpr int kRowsPerBlock = kThreads / kRowLanes;   // 8
+constexpr int kGrid = 1024;                            // blocks of every reducing kernel
 
-template <typename T>
-__device__ __forceinline__ T shfl_xor(T v, int mask) {
-  return __shfl_xor(v, mask, 64);
-}
-
-// Sum of a double over the block (fixed tree), result valid in thread 0.
+// Sum of a double over the block (fixed tree); every thread gets the result.
 __device__ double block_sum(double v, double* red) {
-  const int tid = threadIdx.x;
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  const int w = tid >> 6, lane = tid & 63;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  __syncthreads();
   if (lane == 0) red[w] = v;
   __syncthreads();
   double s = 0.0;
-  if (tid == 0) {
-    const int nw = blockDim.x >> 6;
-    for (int q = 0; q < nw; ++q) s += red[q];
-  }
-  __syncthreads();
+  const int nw = blockDim.x >> 6;
+  for (int q = 0; q < nw; ++q) s += red[q];
   return s;
 }
 
-// Writes `val` as this block's partial; the last block sums all partials in
-// index order into *out.  ticket must be zero on entry and is left zero.
-__device__ bool finalize_sum(double val, double* partials, unsigned* ticket, double* red,
-                             double* out) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = val;
-    __threadfence();
-    const unsigned prev = atomicAdd(ticket, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+// Deterministic sum of kGrid partials (same order in every block).
+__device__ double sum_partials(const double* __restrict__ part, double* red) {
   double s = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-    s += __hip_atomic_load(&partials[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = block_sum(s, red);
-  if (threadIdx.x == 0) {
-    *out = s;
-    *ticket = 0u;
-  }
-  return true;
+  for (int b = threadIdx.x; b < kGrid; b += blockDim.x) s += part[b];
+  return block_sum(s, red);
 }
 
-template <typename T>
-__device__ __forceinline__ void row_product(const fem_mesh& m, const T* __restrict__ vals,
-                                            const T* __restrict__ x, int64_t i, int lane,
-                                            T& a0, T& a1, T& a2) {
-  a0 = a1 = a2 = T(0);
-  if (i < m.num_node) {
-    const int64_t sm = m.sm_node[i];
-    const int nv = 9 * m.co_node[i];
-    const T* __restrict__ v = vals + 9 * sm;
-    const int32_t* __restrict__ cols = m.re_node + sm;
-    for (int t = lane; t < nv; t += kRowLanes) {
-      const int b = t / 9, r = t - 9 * b;
-      const int p = r / 3, q = r - 3 * p;
-      const T prod = v[t] * x[3 * (int64_t)cols[b] + q];
-      a0 += (p == 0) ? prod : T(0);
-      a1 += (p == 1) ? prod : T(0);
-      a2 += (p == 2) ? prod : T(0);
-    }
+template <typename VT, typename XT>
+__device__ __forceinline__ void row_product(const fem_mesh& m, const VT* __restrict__ vals,
+                                            const XT* __restrict__ x, int64_t i, int lane,
+                                            XT& a0, XT& a1, XT& a2) {
+  a0 = a1 = a2 = XT(0);
+  const int64_t sm = m.sm_node[i];
+  const int co = m.co_node[i];
+  for (int b = lane; b < co; b += kRowLanes) {
+    const VT* __restrict__ v = vals + 9 * (sm + b);
+    const int64_t c = 3 * (int64_t)m.re_node[sm + b];
+    const XT x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+    a0 += (XT)v[0] * x0 + (XT)v[1] * x1 + (XT)v[2] * x2;
+    a1 += (XT)v[3] * x0 + (XT)v[4] * x1 + (XT)v[5] * x2;
+    a2 += (XT)v[6] * x0 + (XT)v[7] * x1 + (XT)v[8] * x2;
   }
 #pragma unroll
   for (int o = kRowLanes / 2; o > 0; o >>= 1) {
-    a0 += shfl_xor(a0, o);
-    a1 += shfl_xor(a1, o);
-    a2 += shfl_xor(a2, o);
+    a0 += __shfl_xor(a0, o, 64);
+    a1 += __shfl_xor(a1, o, 64);
+    a2 += __shfl_xor(a2, o, 64);
   }
 }
 
-// y = A x; optionally partial x.y for the PCG step (state[1] = pq,
-// state[2] = alpha = state[0] / pq).
-template <typename T, bool DOT>
-__global__ void __launch_bounds__(kSpmvThreads)
-spmv_kernel(fem_mesh m, const T* __restrict__ vals, const T* __restrict__ x, T* __restrict__ y,
-            double* partials, unsigned* ticket, double* state) {
-  __shared__ double red[kSpmvThreads / 64];
+// y = A x (grid-stride over rows).  With DOT, partial x.y per block into
+// part[blockIdx.x]; block 0 also moves state[1] (rz of the previous
+// iteration) into state[0] for the update kernel that follows.
+template <typename VT, typename XT, bool DOT>
+__global__ void __launch_bounds__(kThreads)
+spmv_kernel(fem_mesh m, const VT* __restrict__ vals, const XT* __restrict__ x, XT* __restrict__ y,
+            double* __restrict__ part, double* state) {
+  __shared__ double red[kThreads / 64];
   const int lane = threadIdx.x & (kRowLanes - 1);
-  const int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + (threadIdx.x / kRowLanes);
-  T a0, a1, a2;
-  row_product(m, vals, x, i, lane, a0, a1, a2);
   double d = 0.0;
-  if (i < m.num_node && lane == 0) {
-    y[3 * i] = a0;
-    y[3 * i + 1] = a1;
-    y[3 * i + 2] = a2;
-    if (DOT) d = (double)a0 * (double)x[3 * i] + (double)a1 * (double)x[3 * i + 1] +
-                 (double)a2 * (double)x[3 * i + 2];
+  const int64_t stride = (int64_t)gridDim.x * kRowsPerBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kRowLanes; i < m.num_node;
+       i += stride) {
+    XT a0, a1, a2;
+    row_product(m, vals, x, i, lane, a0, a1, a2);
+    if (lane == 0) {
+      y[3 * i] = a0;
+      y[3 * i + 1] = a1;
+      y[3 * i + 2] = a2;
+      if (DOT) d += (double)a0 * (double)x[3 * i] + (double)a1 * (double)x[3 * i + 1] +
+                    (double)a2 * (double)x[3 * i + 2];
+    }
   }
   if (DOT) {
     d = block_sum(d, red);
-    double pq;
-    if (finalize_sum(d, partials, ticket, red, &state[1])) {
-      if (threadIdx.x == 0) {
-        pq = state[1];
-        state[2] = state[0] / pq;
-      }
+    if (threadIdx.x == 0) {
+      part[blockIdx.x] = d;
+      if (blockIdx.x == 0) state[0] = state[1];
     }
   }
 }
@@ -193,23 +166,22 @@ __global__ void bj_setup_kernel(fem_mesh m, const T* __restrict__ vals, T* __res
   for (int q = 0; q < 9; ++q) dinv[9 * i + q] = (T)Bi[q];
 }
 
-// r = b - q; z = Dinv r; p = z; partial sums of r.z, r.r, b.b.  q may alias z
-// (each thread reads its own entries before writing them).
-template <typename T>
-__global__ void __launch_bounds__(kVecThreads)
-pcg_start_kernel(int64_t n, const T* __restrict__ b, const T* q, const T* __restrict__ dinv,
-                 T* r, T* z, T* p, double* partials,
-                 unsigned* tickets, double* state) {
-  __shared__ double red[kVecThreads / 64];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// r = b - q; z = Dinv r; p = z; partials of r.z, r.r, b.b (q may alias z:
+// each thread reads its own entries before writing them).
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(kThreads)
+pcg_start_kernel(int64_t N, const XT* __restrict__ b, const XT* q, const VT* __restrict__ dinv,
+                 XT* r, XT* z, XT* p, double* __restrict__ part) {
+  __shared__ double red[kThreads / 64];
   double rz = 0, rr = 0, bb = 0;
-  if (i < n) {
-    T ri[3], zi[3];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    XT ri[3], zi[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) ri[a] = b[3 * i + a] - q[3 * i + a];
-    const T* D = dinv + 9 * i;
+    const VT* D = dinv + 9 * i;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) zi[a] = D[3 * a] * ri[0] + D[3 * a + 1] * ri[1] + D[3 * a + 2] * ri[2];
+    for (int a = 0; a < 3; ++a)
+      zi[a] = (XT)D[3 * a] * ri[0] + (XT)D[3 * a + 1] * ri[1] + (XT)D[3 * a + 2] * ri[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       r[3 * i + a] = ri[a];
@@ -221,37 +193,57 @@ pcg_start_kernel(int64_t n, const T* __restrict__ b, const T* q, const T* __rest
       bb += bv * bv;
     }
   }
-  const int G = gridDim.x;
   rz = block_sum(rz, red);
-  finalize_sum(rz, partials, tickets, red, &state[0]);
   rr = block_sum(rr, red);
-  finalize_sum(rr, partials + G, tickets + 1, red, &state[4]);
   bb = block_sum(bb, red);
-  finalize_sum(bb, partials + 2 * G, tickets + 2, red, &state[5]);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz;
+    part[kGrid + blockIdx.x] = rr;
+    part[2 * kGrid + blockIdx.x] = bb;
+  }
 }
 
-// x += a p; r -= a q; z = Dinv r; partial r.z, r.r -> beta.
-template <typename T>
-__global__ void __launch_bounds__(kVecThreads)
-pcg_update_kernel(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                  const T* __restrict__ p, const T* __restrict__ q, const T* __restrict__ dinv,
-                  double* partials, unsigned* tickets, double* state) {
-  __shared__ double red[kVecThreads / 64];
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double alpha = state[2];
-  const T a = (T)alpha;
+// One block: state[1] = rz (as "next"), state[4] = rr, state[5] = bb.
+__global__ void pcg_start_finalize(const double* __restrict__ part, double* state, int with_bb) {
+  __shared__ double red[kThreads / 64];
+  const double rz = sum_partials(part, red);
+  const double rr = sum_partials(part + kGrid, red);
+  const double bb = sum_partials(part + 2 * kGrid, red);
+  if (threadIdx.x == 0) {
+    state[1] = rz;
+    state[4] = rr;
+    if (with_bb) state[5] = bb;
+  }
+}
+
+// alpha = state[0] / sum(part_pq); x += alpha p; r -= alpha q; z = Dinv r;
+// partials of r.z and r.r.
+template <typename VT, typename XT>
+__global__ void __launch_bounds__(kThreads)
+pcg_update_kernel(int64_t N, XT* __restrict__ x, XT* __restrict__ r, XT* __restrict__ z,
+                  const XT* __restrict__ p, const XT* __restrict__ q, const VT* __restrict__ dinv,
+                  const double* __restrict__ part_pq, double* __restrict__ part_out, double* state) {
+  __shared__ double red[kThreads / 64];
+  const double pq = sum_partials(part_pq, red);
+  const double alpha = (pq != 0.0) ? state[0] / pq : 0.0;   // p = 0 after exact convergence
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    state[2] = alpha;
+    state[3] = pq;
+  }
+  const XT a = (XT)alpha;
   double rz = 0, rr = 0;
-  if (i < n) {
-    T ri[3], zi[3];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    XT ri[3], zi[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       x[3 * i + c] += a * p[3 * i + c];
       ri[c] = r[3 * i + c] - a * q[3 * i + c];
       r[3 * i + c] = ri[c];
     }
-    const T* D = dinv + 9 * i;
+    const VT* D = dinv + 9 * i;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) zi[c] = D[3 * c] * ri[0] + D[3 * c + 1] * ri[1] + D[3 * c + 2] * ri[2];
+    for (int c = 0; c < 3; ++c)
+      zi[c] = (XT)D[3 * c] * ri[0] + (XT)D[3 * c + 1] * ri[1] + (XT)D[3 * c + 2] * ri[2];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       z[3 * i + c] = zi[c];
@@ -259,26 +251,66 @@ pcg_update_kernel(int64_t n, T* __restrict__ x, T* __restrict__ r, T* __restrict
       rr += (double)ri[c] * (double)ri[c];
     }
   }
-  const int G = gridDim.x;
   rz = block_sum(rz, red);
-  double rz_old = state[0];
-  if (finalize_sum(rz, partials, tickets, red, &state[6])) {
-    if (threadIdx.x == 0) {
-      state[3] = state[6] / rz_old;
-      state[0] = state[6];
-    }
-  }
   rr = block_sum(rr, red);
-  finalize_sum(rr, partials + G, tickets + 1, red, &state[4]);
+  if (threadIdx.x == 0) {
+    part_out[blockIdx.x] = rz;
+    part_out[kGrid + blockIdx.x] = rr;
+  }
 }
 
-template <typename T>
-__global__ void pcg_direction_kernel(int64_t n3, const T* __restrict__ z, T* __restrict__ p,
-                                     const double* state) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n3) return;
-  const T beta = (T)state[3];
-  p[t] = z[t] + beta * p[t];
+// beta = sum(part_rz) / state[0]; p = z + beta p; block 0 records
+// state[1] = rz_new (moved to state[0] by the next SpMV), state[4] = rr.
+template <typename XT>
+__global__ void __launch_bounds__(kThreads)
+pcg_direction_kernel(int64_t n3, const XT* __restrict__ z, XT* __restrict__ p,
+                     const double* __restrict__ part, double* state) {
+  __shared__ double red[kThreads / 64];
+  const double rz_new = sum_partials(part, red);
+  const double beta = (state[0] != 0.0) ? rz_new / state[0] : 0.0;
+  if (blockIdx.x == 0) {
+    const double rr = sum_partials(part + kGrid, red);
+    if (threadIdx.x == 0) {
+      state[1] = rz_new;
+      state[4] = rr;
+      state[6] = beta;
+    }
+  }
+  const XT bt = (XT)beta;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n3; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = z[t] + bt * p[t];
+}
+
+// r = b - y; z = Dinv r; partials r.z, r.r (the bb slot is left at 0).
+template <typename VT>
+__global__ void __launch_bounds__(kThreads)
+replace_kernel(int64_t N, const double* __restrict__ b, const double* __restrict__ y,
+               const VT* __restrict__ dinv, double* __restrict__ r, double* __restrict__ z,
+               double* __restrict__ p, double* __restrict__ part) {
+  __shared__ double red[kThreads / 64];
+  double rz = 0, rr = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N; i += (int64_t)gridDim.x * blockDim.x) {
+    double ri[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ri[a] = b[3 * i + a] - y[3 * i + a];
+    const VT* D = dinv + 9 * i;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double za = (double)D[3 * a] * ri[0] + (double)D[3 * a + 1] * ri[1] + (double)D[3 * a + 2] * ri[2];
+      r[3 * i + a] = ri[a];
+      z[3 * i + a] = za;
+      if (p) p[3 * i + a] = za;
+      rz += ri[a] * za;
+      rr += ri[a] * ri[a];
+    }
+  }
+  rz = block_sum(rz, red);
+  rr = block_sum(rr, red);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz;
+    part[kGrid + blockIdx.x] = rr;
+    part[2 * kGrid + blockIdx.x] = 0.0;
+  }
 }
 
 int check_solve_mesh(const fem_mesh* m) {
@@ -289,21 +321,9 @@ int check_solve_mesh(const fem_mesh* m) {
   return FEM_OK;
 }
 
-struct PcgWork {
-  double* partials;
-  unsigned* tickets;
-};
-
-int64_t vec_blocks(int64_t N) { return ceil_div(N > 0 ? N : 1, kVecThreads); }
-int64_t spmv_blocks(int64_t N) { return ceil_div(N > 0 ? N : 1, kRowsPerBlock); }
-
-PcgWork pcg_work(void* work, int64_t N) {
-  int64_t nb = spmv_blocks(N) > vec_blocks(N) ? spmv_blocks(N) : vec_blocks(N);
-  PcgWork w;
-  w.tickets = reinterpret_cast<unsigned*>(work);
-  w.partials = reinterpret_cast<double*>(reinterpret_cast<char*>(work) + 256);
-  (void)nb;
-  return w;
+int64_t grid_for(int64_t items, int per_block) {
+  int64_t g = ceil_div(items > 0 ? items : 1, per_block);
+  return g < kGrid ? g : kGrid;
 }
 
 }  // namespace
@@ -312,8 +332,8 @@ PcgWork pcg_work(void* work, int64_t N) {
 using namespace fem;
 
 extern "C" int64_t fem_pcg_work_bytes(int64_t num_node) {
-  int64_t nb = spmv_blocks(num_node) > vec_blocks(num_node) ? spmv_blocks(num_node) : vec_blocks(num_node);
-  return 256 + 3 * 8 * nb + 256;
+  (void)num_node;
+  return 8 * (int64_t)kGrid * 6 + 256;
 }
 
 extern "C" int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, const void* x, void* y,
@@ -323,13 +343,16 @@ extern "C" int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, c
   if (mesh->num_node == 0) return FEM_OK;
   FEM_REQUIRE(vals && x && y, FEM_E_INVALID, "null pointer");
   hipStream_t st = as_stream(stream);
-  const int64_t nb = spmv_blocks(mesh->num_node);
+  const int64_t nb = ceil_div(mesh->num_node, kRowsPerBlock);
   if (dtype == FEM_DTYPE_F64)
-    spmv_kernel<double, false><<<nb, kSpmvThreads, 0, st>>>(
-        *mesh, (const double*)vals, (const double*)x, (double*)y, nullptr, nullptr, nullptr);
+    spmv_kernel<double, double, false><<<nb, kThreads, 0, st>>>(*mesh, (const double*)vals, (const double*)x,
+                                                                (double*)y, nullptr, nullptr);
   else if (dtype == FEM_DTYPE_F32)
-    spmv_kernel<float, false><<<nb, kSpmvThreads, 0, st>>>(
-        *mesh, (const float*)vals, (const float*)x, (float*)y, nullptr, nullptr, nullptr);
+    spmv_kernel<float, float, false><<<nb, kThreads, 0, st>>>(*mesh, (const float*)vals, (const float*)x,
+                                                              (float*)y, nullptr, nullptr);
+  else if (dtype == FEM_DTYPE_F32F64)
+    spmv_kernel<float, double, false><<<nb, kThreads, 0, st>>>(*mesh, (const float*)vals, (const double*)x,
+                                                               (double*)y, nullptr, nullptr);
   else
     FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype");
   FEM_LAUNCH_CHECK();
@@ -380,34 +403,36 @@ extern "C" int fem_bj_setup(const fem_mesh* mesh, const void* vals, int dtype, v
   return FEM_OK;
 }
 
-template <typename T>
-static int pcg_start_t(const fem_mesh* mesh, const T* vals, const T* dinv, const T* b, T* x, T* r, T* z,
-                       T* p, double* state, void* work, hipStream_t st) {
+template <typename VT, typename XT>
+static int pcg_start_t(const fem_mesh* mesh, const VT* vals, const VT* dinv, const XT* b, XT* x, XT* r, XT* z,
+                       XT* p, double* state, double* part, int with_bb, hipStream_t st) {
   const int64_t N = mesh->num_node;
-  PcgWork w = pcg_work(work, N);
-  FEM_CHECK_HIP(hipMemsetAsync(w.tickets, 0, 256, st));
-  // q := A x (into z as temporary), then r = b - q etc.
-  spmv_kernel<T, false><<<spmv_blocks(N), kSpmvThreads, 0, st>>>(*mesh, vals, x, z, nullptr, nullptr,
-                                                                  nullptr);
-  pcg_start_kernel<T><<<vec_blocks(N), kVecThreads, 0, st>>>(N, b, z, dinv, r, z, p, w.partials,
-                                                            w.tickets, state);
+  // z := A x as a temporary, then r = b - z etc.
+  spmv_kernel<VT, XT, false><<<ceil_div(N, kRowsPerBlock), kThreads, 0, st>>>(*mesh, vals, x, z, nullptr,
+                                                                              nullptr);
+  pcg_start_kernel<VT, XT><<<kGrid, kThreads, 0, st>>>(N, b, z, dinv, r, z, p, part);
+  pcg_start_finalize<<<1, kThreads, 0, st>>>(part, state, with_bb);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
 
-template <typename T>
-static int pcg_iterate_t(const fem_mesh* mesh, const T* vals, const T* dinv, T* x, T* r, T* z, T* p,
-                         T* q, double* state, void* work, hipStream_t st) {
+template <typename VT, typename XT>
+static int pcg_iterate_t(const fem_mesh* mesh, const VT* vals, const VT* dinv, XT* x, XT* r, XT* z, XT* p,
+                         XT* q, double* state, double* part, hipStream_t st) {
   const int64_t N = mesh->num_node;
-  PcgWork w = pcg_work(work, N);
-  spmv_kernel<T, true><<<spmv_blocks(N), kSpmvThreads, 0, st>>>(*mesh, vals, p, q, w.partials,
-                                                                 w.tickets, state);
-  pcg_update_kernel<T><<<vec_blocks(N), kVecThreads, 0, st>>>(N, x, r, z, p, q, dinv, w.partials,
-                                                             w.tickets + 4, state);
-  pcg_direction_kernel<T><<<ceil_div(3 * N, 256), 256, 0, st>>>(3 * N, z, p, state);
+  spmv_kernel<VT, XT, true><<<kGrid, kThreads, 0, st>>>(*mesh, vals, p, q, part, state);
+  pcg_update_kernel<VT, XT><<<kGrid, kThreads, 0, st>>>(N, x, r, z, p, q, dinv, part,
+                                                                       part + kGrid, state);
+  pcg_direction_kernel<XT><<<grid_for(3 * N, kThreads), kThreads, 0, st>>>(3 * N, z, p, part + kGrid, state);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
+
+#define FEM_PCG_DISPATCH(FN, ...)                                                                  \
+  if (dtype == FEM_DTYPE_F64) return FN<double, double>(__VA_ARGS__);                             \
+  if (dtype == FEM_DTYPE_F32) return FN<float, float>(__VA_ARGS__);                               \
+  if (dtype == FEM_DTYPE_F32F64) return FN<float, double>(__VA_ARGS__);                           \
+  FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype %d", dtype);
 
 extern "C" int fem_pcg_start(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
                              const void* b, void* x, void* r, void* z, void* p, double* state,
@@ -418,13 +443,17 @@ extern "C" int fem_pcg_start(const fem_mesh* mesh, const void* vals, const void*
   FEM_REQUIRE(vals && dinv && b && x && r && z && p && state && work, FEM_E_INVALID, "null pointer");
   FEM_REQUIRE(work_bytes >= fem_pcg_work_bytes(mesh->num_node), FEM_E_INVALID, "work too small");
   hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
   if (dtype == FEM_DTYPE_F64)
-    return pcg_start_t<double>(mesh, (const double*)vals, (const double*)dinv, (const double*)b,
-                               (double*)x, (double*)r, (double*)z, (double*)p, state, work, st);
+    return pcg_start_t<double, double>(mesh, (const double*)vals, (const double*)dinv, (const double*)b,
+                                       (double*)x, (double*)r, (double*)z, (double*)p, state, part, 1, st);
   if (dtype == FEM_DTYPE_F32)
-    return pcg_start_t<float>(mesh, (const float*)vals, (const float*)dinv, (const float*)b, (float*)x,
-                              (float*)r, (float*)z, (float*)p, state, work, st);
-  FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype");
+    return pcg_start_t<float, float>(mesh, (const float*)vals, (const float*)dinv, (const float*)b, (float*)x,
+                                     (float*)r, (float*)z, (float*)p, state, part, 1, st);
+  if (dtype == FEM_DTYPE_F32F64)
+    return pcg_start_t<float, double>(mesh, (const float*)vals, (const float*)dinv, (const double*)b,
+                                      (double*)x, (double*)r, (double*)z, (double*)p, state, part, 1, st);
+  FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype %d", dtype);
 }
 
 extern "C" int fem_pcg_iterate(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
@@ -436,11 +465,44 @@ extern "C" int fem_pcg_iterate(const fem_mesh* mesh, const void* vals, const voi
   FEM_REQUIRE(vals && dinv && x && r && z && p && q && state && work, FEM_E_INVALID, "null pointer");
   FEM_REQUIRE(work_bytes >= fem_pcg_work_bytes(mesh->num_node), FEM_E_INVALID, "work too small");
   hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
   if (dtype == FEM_DTYPE_F64)
-    return pcg_iterate_t<double>(mesh, (const double*)vals, (const double*)dinv, (double*)x, (double*)r,
-                                 (double*)z, (double*)p, (double*)q, state, work, st);
+    return pcg_iterate_t<double, double>(mesh, (const double*)vals, (const double*)dinv, (double*)x,
+                                         (double*)r, (double*)z, (double*)p, (double*)q, state, part, st);
   if (dtype == FEM_DTYPE_F32)
-    return pcg_iterate_t<float>(mesh, (const float*)vals, (const float*)dinv, (float*)x, (float*)r,
-                                (float*)z, (float*)p, (float*)q, state, work, st);
-  FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype");
+    return pcg_iterate_t<float, float>(mesh, (const float*)vals, (const float*)dinv, (float*)x, (float*)r,
+                                       (float*)z, (float*)p, (float*)q, state, part, st);
+  if (dtype == FEM_DTYPE_F32F64)
+    return pcg_iterate_t<float, double>(mesh, (const float*)vals, (const float*)dinv, (double*)x, (double*)r,
+                                        (double*)z, (double*)p, (double*)q, state, part, st);
+  FEM_REQUIRE(false, FEM_E_INVALID, "bad dtype %d", dtype);
+}
+
+/* Residual replacement for the mixed form: r = b - y (y = A64 x computed by
+ * the caller), z = Dinv r, state[1] = r.z (becomes rz at the next SpMV),
+ * state[4] = r.r; with p_restart non-NULL also p = z (CG restart). */
+extern "C" int fem_pcg_replace_residual(const fem_mesh* mesh, const void* dinv, int dtype, const void* b,
+                                        const void* y, void* r, void* z, void* p_restart, double* state,
+                                        void* work, int64_t work_bytes, void* stream) {
+  int rc = check_solve_mesh(mesh);
+  if (rc) return rc;
+  if (mesh->num_node == 0) return FEM_OK;
+  FEM_REQUIRE(dtype == FEM_DTYPE_F32F64 || dtype == FEM_DTYPE_F64, FEM_E_INVALID, "bad dtype");
+  FEM_REQUIRE(dinv && b && y && r && z && state && work, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(work_bytes >= fem_pcg_work_bytes(mesh->num_node), FEM_E_INVALID, "work too small");
+  hipStream_t st = as_stream(stream);
+  const int64_t N = mesh->num_node;
+  double* part = reinterpret_cast<double*>(work) + 3 * kGrid;
+  // r = b - y; z = Dinv r (p untouched: the start kernel writes p = z, so use a scratch p = r)
+  if (dtype == FEM_DTYPE_F32F64)
+    replace_kernel<float><<<kGrid, kThreads, 0, st>>>(N, (const double*)b, (const double*)y,
+                                                                     (const float*)dinv, (double*)r, (double*)z,
+                                                                     (double*)p_restart, part);
+  else
+    replace_kernel<double><<<kGrid, kThreads, 0, st>>>(N, (const double*)b, (const double*)y,
+                                                                      (const double*)dinv, (double*)r,
+                                                                      (double*)z, (double*)p_restart, part);
+  pcg_start_finalize<<<1, kThreads, 0, st>>>(part, state, 0);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
 }

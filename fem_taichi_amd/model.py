@@ -109,10 +109,15 @@ class Model:
         _lib.call("fem_hex8_assemble", self.dev.ref, self._young_ptr(), self.young_scalar, float(self.PO),
                   float(self.DE), vals.data_ptr(), cdt, lay, None, st)
         if penalty:
-            nb = int(self.bound_node.numel())
-            _lib.call("fem_apply_penalty", self.dev.ref, self.bound_node.data_ptr(),
-                      self.bound_flag.data_ptr(), nb, vals.data_ptr(), cdt, lay, PENALTY, st)
+            self.apply_penalty(vals, dtype, layout)
         return vals
+
+    def apply_penalty(self, vals, dtype="f64", layout="bsr"):
+        """ModifyK (FEM_model_csr.py:233-242): SET constrained diagonals to 1e20."""
+        cdt = _lib.DTYPE_F64 if dtype == "f64" else _lib.DTYPE_F32
+        lay = _lib.LAYOUT_BSR if layout == "bsr" else _lib.LAYOUT_CSR
+        _lib.call("fem_apply_penalty", self.dev.ref, self.bound_node.data_ptr(), self.bound_flag.data_ptr(),
+                  int(self.bound_node.numel()), vals.data_ptr(), cdt, lay, PENALTY, stream_ptr())
 
     def M_blocks(self):
         """Mass coupling scalar of every node-pair block (nnzb,), device f64."""

@@ -20,6 +20,9 @@ from . import _lib
 from .groups import DEFAULT_BRICK, build_row_groups, node_group_ids
 from .runtime import device, stream_ptr
 
+# Upper bound on the elements one row group may touch (LDS staging budget).
+MAX_GROUP_ELEMS = 48
+
 
 class DeviceMesh:
     """Device-resident mesh, pattern, incidences and row groups + the C descriptor."""
@@ -44,8 +47,10 @@ class DeviceMesh:
         m.num_groups = int(groups["num_groups"])
         m.max_group_elems = int(groups["max_group_elems"])
         m.max_group_rows = int(groups["max_group_rows"])
+        m.max_group_slots = int(groups["max_group_slots"])
         self.c = m
-        self.groups_info = {k: groups[k] for k in ("num_groups", "max_group_elems", "max_group_rows")}
+        self.groups_info = {k: groups[k] for k in ("num_groups", "max_group_elems", "max_group_rows",
+                                                   "max_group_slots")}
 
     @property
     def ref(self):
@@ -126,11 +131,11 @@ class Physics:
         ne_list = pat["ne_list"].cpu().numpy()
         coords = np.ascontiguousarray(np.asarray(self.Node[-3:], dtype=np.float64).T)
         gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=self.brick)
-        lds_cap = 160 * 1024
-        max_elems = 48
-        while (max_elems < 4096 and
-               _lib.lib().fem_hex8_assemble_lds_bytes(max_elems * 2, _lib.DTYPE_F64, 1, 1) <= lds_cap // 2):
-            max_elems *= 2
-        groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, max_elems)
+        groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, MAX_GROUP_ELEMS,
+                                  co_node=self.CoNode)
+        lds = _lib.lib().fem_hex8_assemble_lds_bytes(groups["max_group_elems"], groups["max_group_rows"],
+                                                      groups["max_group_slots"], _lib.DTYPE_F64, 1, 1)
+        if lds > 160 * 1024:
+            raise ValueError(f"row groups need {lds} B of LDS; use a smaller brick")
         self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
         return self

@@ -39,6 +39,7 @@ extern "C" {
 
 #define FEM_DTYPE_F64 0
 #define FEM_DTYPE_F32 1
+#define FEM_DTYPE_F32F64 2   /* FP32 matrix values, FP64 vectors (solver kernels) */
 
 #define FEM_LAYOUT_BSR 0     /* vals[9*s + 3*p + q]                              */
 #define FEM_LAYOUT_CSR 1     /* the reference's CSR: 9*Sm[i] + 3*p*Co[i] + 3*m + q
@@ -73,6 +74,8 @@ typedef struct fem_mesh {
   int64_t num_groups;
   int32_t max_group_elems;     /* max over groups of the element count */
   int32_t max_group_rows;      /* max over groups of the row count     */
+  int32_t max_group_slots;     /* max over groups of sum CoNode of its rows */
+  int32_t pad0;
   const int32_t* grp_row_ptr;  /* G + 1 */
   const int32_t* grp_rows;     /* N, rows in group order */
   const int32_t* grp_elem_ptr; /* G + 1 */
@@ -124,9 +127,10 @@ int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_sc
                       double nu, double rho, void* k_vals, int k_dtype, int k_layout,
                       double* m_blocks, void* stream);
 
-/* LDS bytes one work-group of fem_hex8_assemble uses for a group of
- * max_group_elems elements (to check a grouping fits 160 KB before launch). */
-int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int k_dtype, int want_k, int want_m);
+/* LDS bytes one work-group of fem_hex8_assemble uses for a row group of the
+ * given sizes (to check a grouping fits the 160 KB of a CU before launch). */
+int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
+                                    int32_t max_group_slots, int k_dtype, int want_k, int want_m);
 
 /* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and
  *      ModifyMatrixK FEM_model_bsr.py:248-259) ------------------------------
@@ -161,8 +165,8 @@ int fem_stress_recover(const fem_mesh* mesh, const double* young, double young_s
 /* ---- SpMV / SpMM on the BSR(3x3) pattern (the K/M products inside
  *      Lanczos_solve SOLVE_eigen.py:65-74 and the PCG forms) -------------- */
 
-/* y = A x for the 3x3-block matrix (sm_node, re_node, vals); dtype f64 or
- * f32 (x, y of the same dtype). */
+/* y = A x for the 3x3-block matrix (sm_node, re_node, vals); dtype F64,
+ * F32 (x, y of the value dtype) or F32F64 (f32 values, f64 x and y). */
 int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, const void* x, void* y,
                  void* stream);
 /* Y = A X for nrhs column-interleaved right-hand sides X[(3*i+p)*nrhs + c]. */
@@ -184,17 +188,25 @@ int fem_bj_setup(const fem_mesh* mesh, const void* vals, int dtype, void* dinv,
 /* Scratch bytes of the fused PCG kernels for n3 = 3N unknowns. */
 int64_t fem_pcg_work_bytes(int64_t num_node);
 
-/* The PCG scalar state lives in device memory (8 doubles): state[0]=rz,
- * [1]=pq, [2]=alpha, [3]=beta, [4]=rr, [5]=bb, [6]=scratch. */
+/* The PCG scalar state lives in device memory (8 doubles): state[0] = rz
+ * used by the current iteration, [1] = rz of the latest residual, [2] =
+ * alpha, [3] = p.q, [4] = r.r, [5] = b.b, [6] = beta. */
 /* r = b - A x; z = Dinv r; p = z; rz, rr, bb. */
 int fem_pcg_start(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
                   const void* b, void* x, void* r, void* z, void* p, double* state,
                   void* work, int64_t work_bytes, void* stream);
 /* One iteration: q = A p; alpha = rz / p.q; x += alpha p; r -= alpha q;
- * z = Dinv r; beta = r.z / rz; p = z + beta p; rr = r.r. */
+ * z = Dinv r; beta = r.z / rz; p = z + beta p; rr = r.r.  Dot products are
+ * reduced in a fixed order without atomics (bit-reproducible). */
 int fem_pcg_iterate(const fem_mesh* mesh, const void* vals, const void* dinv, int dtype,
                     void* x, void* r, void* z, void* p, void* q, double* state,
                     void* work, int64_t work_bytes, void* stream);
+/* Residual replacement (FP64 vectors): r = b - y where y = A64 x was
+ * computed by the caller, z = Dinv r, rz and rr refreshed in state; when
+ * p_restart is non-NULL the search direction restarts, p = z. */
+int fem_pcg_replace_residual(const fem_mesh* mesh, const void* dinv, int dtype, const void* b,
+                             const void* y, void* r, void* z, void* p_restart, double* state,
+                             void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }
