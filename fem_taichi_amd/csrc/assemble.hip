@@ -5,19 +5,20 @@
 // computes an E x 8 x 8 x 3 x 3 element intermediate and scatters it with
 // atomic += after a linear slot search.  Here one work-group owns a group of
 // node rows (a spatially compact set built once from the pattern):
-//   phase 1: it stages in LDS the physical shape-function gradients
-//            a[e][k][g] = dN_k/dx at Gauss point g of every element touching
-//            its rows and w[e][g] = E_e * detJ_g (one thread per (e, g));
+//   phase 0: the group's row list, neighbour lists and incidence lists are
+//            staged in LDS (so the rounds below touch no global memory);
+//   phase 1: one thread per (element, Gauss point) of the elements touching
+//            the rows computes J, stores J^-1 and w = E_e*detJ in LDS;
 //   phase 2: 16 lanes own one row: lane = (local column node k, half h of
 //            the Gauss points).  Round t processes the row's t-th incidence
 //            (element e, local row node j) in ascending element order: the
-//            lane forms the 3x3 Gram block sum_g w_g a_gj (x) a_gk over its
-//            four Gauss points, the two halves are combined by a lane
-//            shuffle, and the block is added to the row's LDS accumulator at
-//            the slot of column node k.  Within a round the 8 lanes of a row
-//            hit 8 distinct slots; rounds are ordered, so every block is a sum
-//            in ascending element order -- bit-identical run to run, no
-//            atomics;
+//            lane forms the physical gradients a_gj = DNr_g[j] J_g^-1 and
+//            a_gk, accumulates the 3x3 Gram block sum_g w_g a_gj (x) a_gk over
+//            its four Gauss points, combines the two halves with a lane
+//            shuffle and adds the block to the row's LDS accumulator at the
+//            slot of column node k.  Within a round the 8 lanes of a row hit
+//            8 distinct slots; rounds are ordered, so every block is a sum in
+//            ascending element order -- bit-identical run to run, no atomics;
 //   phase 3: every block of the group's rows is written to HBM once,
 //            coalesced, as K_block = lhat*G + mhat*G^T + mhat*tr(G)*I.
 // Phase 3's closed form is B_j^T D B_k with D = E*[lhat 1 1^T + 2 mhat I on
@@ -32,26 +33,28 @@ namespace {
 constexpr int kAsmThreads = 256;
 constexpr int kLanesPerRow = 16;
 constexpr int kRowsPerPass = kAsmThreads / kLanesPerRow;
-constexpr int kAStride = 25;   // (e, k) stride of a[], in elements of T: 24 used + 1 pad (LDS banks)
+
+template <typename T>
+struct JStride {   // per (element, Gauss point): J^-1 (9) + w (1), padded for 16-B loads
+  static constexpr int value = (sizeof(T) == 8) ? 10 : 12;
+};
 
 struct AsmLds {
-  int a_off, w_off, wm_off, n_off, nodes_off, dup_off, rows_off, rowpre_off, re_off, acc_off, accm_off, total;
+  int ji_off, acc_off, wm_off, accm_off, n_off, dnr_off, nodes_off, dup_off, rows_off, rowpre_off, sm_off,
+      incpre_off, re_off, inc_off, slot_off, total;
 };
 
 __host__ __device__ inline int align16(int o) { return (o + 15) & ~15; }
 
 template <typename T>
-__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, bool want_k, bool want_m) {
+__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, int ig, bool want_k, bool want_m) {
   AsmLds L;
   int o = 0;
-  L.a_off = o;
-  if (want_k) o += eg * 8 * kAStride * (int)sizeof(T);
+  L.ji_off = o;
+  o += eg * 8 * JStride<T>::value * (int)sizeof(T);
   o = align16(o);
   L.acc_off = o;
   if (want_k) o += sg * 9 * (int)sizeof(T);
-  o = align16(o);
-  L.w_off = o;
-  if (want_k) o += eg * 8 * (int)sizeof(T);
   o = align16(o);
   L.wm_off = o;
   if (want_m) o += eg * 8 * (int)sizeof(double);
@@ -61,6 +64,9 @@ __host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, bool wa
   o = align16(o);
   L.n_off = o;
   if (want_m) o += 64 * (int)sizeof(double);
+  L.dnr_off = o;
+  if (want_k) o += 8 * 8 * 4 * (int)sizeof(T);   // DNr[g][k][0..2] padded to 4
+  o = align16(o);
   L.nodes_off = o;
   o += eg * 8 * (int)sizeof(int32_t);
   L.dup_off = o;
@@ -69,8 +75,16 @@ __host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, bool wa
   o += rg * (int)sizeof(int32_t);
   L.rowpre_off = o;
   o += (rg + 1) * (int)sizeof(int32_t);
+  L.sm_off = o;
+  o += rg * (int)sizeof(int32_t);
+  L.incpre_off = o;
+  o += (rg + 1) * (int)sizeof(int32_t);
   L.re_off = o;
   o += sg * (int)sizeof(int32_t);
+  L.inc_off = o;
+  o += ig * (int)sizeof(int32_t);
+  L.slot_off = o;
+  o += ig * 8 * (int)sizeof(int16_t);
   L.total = align16(o);
   return L;
 }
@@ -79,20 +93,25 @@ template <typename T, bool WANT_K, bool WANT_M, int LAYOUT>
 __global__ void __launch_bounds__(kAsmThreads)
 hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar,
                      double lhat, double mhat, double rho, T* __restrict__ k_vals,
-                     double* __restrict__ m_blocks, int eg, int rg, int sg) {
+                     double* __restrict__ m_blocks, int eg, int rg, int sg, int ig) {
+  constexpr int JS = JStride<T>::value;
   extern __shared__ __align__(16) unsigned char smem[];
-  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, WANT_K, WANT_M);
-  T* s_a = reinterpret_cast<T*>(smem + L.a_off);
+  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, ig, WANT_K, WANT_M);
+  T* s_ji = reinterpret_cast<T*>(smem + L.ji_off);
   T* s_acc = reinterpret_cast<T*>(smem + L.acc_off);
-  T* s_w = reinterpret_cast<T*>(smem + L.w_off);
   double* s_wm = reinterpret_cast<double*>(smem + L.wm_off);
   double* s_accm = reinterpret_cast<double*>(smem + L.accm_off);
   double* s_N = reinterpret_cast<double*>(smem + L.n_off);
+  T* s_dnr = reinterpret_cast<T*>(smem + L.dnr_off);
   int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + L.nodes_off);
   int32_t* s_dup = reinterpret_cast<int32_t*>(smem + L.dup_off);
   int32_t* s_rows = reinterpret_cast<int32_t*>(smem + L.rows_off);
   int32_t* s_rowpre = reinterpret_cast<int32_t*>(smem + L.rowpre_off);
+  int32_t* s_sm = reinterpret_cast<int32_t*>(smem + L.sm_off);
+  int32_t* s_incpre = reinterpret_cast<int32_t*>(smem + L.incpre_off);
   int32_t* s_re = reinterpret_cast<int32_t*>(smem + L.re_off);
+  int32_t* s_inc = reinterpret_cast<int32_t*>(smem + L.inc_off);
+  int16_t* s_slot = reinterpret_cast<int16_t*>(smem + L.slot_off);
 
   const int grp = blockIdx.x;
   const int e0 = m.grp_elem_ptr[grp];
@@ -101,21 +120,31 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   const int nr = m.grp_row_ptr[grp + 1] - r0;
   const int tid = threadIdx.x;
 
-  // ---- phase 0: element node lists, rows, shape table
+  // ---- phase 0a: per-row scalars and element node lists (independent loads)
   for (int t = tid; t < ne * 8; t += kAsmThreads) {
     const int e = m.grp_elems[e0 + (t >> 3)];
     s_nodes[t] = m.elem[(int64_t)e * 8 + (t & 7)];
   }
-  for (int t = tid; t < nr; t += kAsmThreads) s_rows[t] = m.grp_rows[r0 + t];
-  if (WANT_M && tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);   // s_N[g*8 + k]
+  for (int r = tid; r < nr; r += kAsmThreads) {
+    const int32_t i = m.grp_rows[r0 + r];
+    s_rows[r] = i;
+    s_rowpre[r + 1] = m.co_node[i];                    // counts; scanned below
+    s_sm[r] = m.sm_node[i];
+    s_incpre[r + 1] = m.ne_ptr[i + 1] - m.ne_ptr[i];
+  }
+  if (WANT_M && tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);          // N[g*8 + k]
+  if (WANT_K && tid < 192) {                                              // DNr[g][k][d]
+    const int g = tid / 24, k = (tid / 3) % 8, d = tid % 3;
+    s_dnr[(g * 8 + k) * 4 + d] = (T)dnr(g, k, d);
+  }
   __syncthreads();
   if (tid == 0) {
-    int acc = 0;
+    s_rowpre[0] = 0;
+    s_incpre[0] = 0;
     for (int r = 0; r < nr; ++r) {
-      s_rowpre[r] = acc;
-      acc += m.co_node[s_rows[r]];
+      s_rowpre[r + 1] += s_rowpre[r];
+      s_incpre[r + 1] += s_incpre[r];
     }
-    s_rowpre[nr] = acc;
   }
   for (int le = tid; le < ne; le += kAsmThreads) {
     int dup = 0;
@@ -125,59 +154,81 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   }
   __syncthreads();
   const int nslots = s_rowpre[nr];
-  // row neighbour lists into LDS, accumulators to zero
+
+  // ---- phase 0b: neighbour and incidence lists into LDS, zero accumulators
   for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
     const int32_t i = s_rows[r];
-    const int32_t sm = m.sm_node[i];
     const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
-    for (int q = tid % kLanesPerRow; q < co; q += kLanesPerRow) s_re[base + q] = m.re_node[sm + q];
+    for (int q = tid % kLanesPerRow; q < co; q += kLanesPerRow) s_re[base + q] = m.re_node[s_sm[r] + q];
+    const int ib = m.ne_ptr[i], b2 = s_incpre[r], ni = s_incpre[r + 1] - b2;
+    for (int q = tid % kLanesPerRow; q < ni; q += kLanesPerRow)
+      s_inc[b2 + q] = (m.ne_local[ib + q] << 3) | (m.ne_list[ib + q] & 7);
   }
   if (WANT_K)
     for (int t = tid; t < nslots * 9; t += kAsmThreads) s_acc[t] = T(0);
   if (WANT_M)
     for (int t = tid; t < nslots; t += kAsmThreads) s_accm[t] = 0.0;
+  __syncthreads();
+  // slot of every (incidence, column k) in its row's sorted neighbour list --
+  // the reference's linear search (FEM_model_csr.py:158-160), done once here
+  {
+    const int ninc = s_incpre[nr];
+    for (int t = tid; t < ninc * 8; t += kAsmThreads) {
+      const int inc = t >> 3, k = t & 7;
+      int r = 0, hi = nr - 1;   // row of incidence inc
+      while (r < hi) {
+        const int mid = (r + hi + 1) >> 1;
+        if (s_incpre[mid] <= inc) r = mid; else hi = mid - 1;
+      }
+      const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
+      const int32_t c = s_nodes[(s_inc[inc] >> 3) * 8 + k];
+      int lo = 0;
+      hi = co - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (s_re[base + mid] < c) lo = mid + 1; else hi = mid;
+      }
+      s_slot[t] = (int16_t)(base + lo);
+    }
+  }
 
-  // ---- phase 1: geometry at each (element, Gauss point)
+  // ---- phase 1: J^-1 and w at each (element, Gauss point)
   for (int t = tid; t < ne * 8; t += kAsmThreads) {
     const int le = t >> 3, g = t & 7;
     const int e = m.grp_elems[e0 + le];
-    double x0[3], J[9];
-    {
-      const int32_t n0 = s_nodes[le * 8];
-      for (int a = 0; a < 3; ++a) x0[a] = m.coords[(int64_t)n0 * 3 + a];
+    double x[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int32_t nk = s_nodes[le * 8 + k];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) x[k][a] = m.coords[(int64_t)nk * 3 + a];
     }
+    double J[9];
+#pragma unroll
     for (int q = 0; q < 9; ++q) J[q] = 0.0;
-    double dn[8][3];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       double f[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) f[a] = 1.0 + node_sign(k, a) * gauss_sign(g, a) * kGS;
-      dn[k][0] = 0.125 * node_sign(k, 0) * f[1] * f[2];
-      dn[k][1] = 0.125 * node_sign(k, 1) * f[0] * f[2];
-      dn[k][2] = 0.125 * node_sign(k, 2) * f[0] * f[1];
-      const int32_t nk = s_nodes[le * 8 + k];
-      double x[3];
+      const double dn[3] = {0.125 * node_sign(k, 0) * f[1] * f[2], 0.125 * node_sign(k, 1) * f[0] * f[2],
+                            0.125 * node_sign(k, 2) * f[0] * f[1]};
 #pragma unroll
-      for (int a = 0; a < 3; ++a) x[a] = m.coords[(int64_t)nk * 3 + a] - x0[a];
+      for (int a = 0; a < 3; ++a) {
+        const double xr = x[k][a] - x[0][a];
 #pragma unroll
-      for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(x[a], dn[k][d], J[a * 3 + d]);
+        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr, dn[d], J[a * 3 + d]);
+      }
     }
     if (WANT_K) {
       double Ji[9];
       const double det = inv3(J, Ji);
       const double Ee = young ? young[e] : young_scalar;
-      s_w[le * 8 + g] = (T)(Ee * det);
+      T* dst = s_ji + (le * 8 + g) * JS;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) dst[q] = (T)Ji[q];
+      dst[9] = (T)(Ee * det);
       if (WANT_M) s_wm[le * 8 + g] = rho * det;
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        T* dst = s_a + (le * 8 + k) * kAStride + 3 * g;
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-          dst[a] = (T)(dn[k][0] * Ji[0 * 3 + a] + dn[k][1] * Ji[1 * 3 + a] + dn[k][2] * Ji[2 * 3 + a]);
-      }
     } else if (WANT_M) {
       s_wm[le * 8 + g] = rho * det3(J);
     }
@@ -188,37 +239,44 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   {
     const int lane = tid % kLanesPerRow;
     const int k = lane >> 1, h = lane & 1;
+    T dk[4][3];   // DNr_g[k] for this lane's four Gauss points
+    if (WANT_K) {
+#pragma unroll
+      for (int gg = 0; gg < 4; ++gg)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) dk[gg][d] = s_dnr[((4 * h + gg) * 8 + k) * 4 + d];
+    }
     for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
-      const int32_t i = s_rows[r];
-      const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
-      const int32_t ib = m.ne_ptr[i], ie = m.ne_ptr[i + 1];
-      for (int32_t inc = ib; inc < ie; ++inc) {
-        const int j = m.ne_list[inc] & 7;
-        const int le = m.ne_local[inc];
-        const int32_t c = s_nodes[le * 8 + k];
-        int lo = 0, hi = co - 1;   // slot of column node c in the row's sorted list
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (s_re[base + mid] < c) lo = mid + 1; else hi = mid;
-        }
-        const int slot = base + lo;
+      const int ib = s_incpre[r], ie = s_incpre[r + 1];
+#pragma unroll 2
+      for (int t = ib; t < ie; ++t) {
+        const int pk = s_inc[t];
+        const int j = pk & 7, le = pk >> 3;
+        const int slot = s_slot[t * 8 + k];
         T G[9];
         double ms = 0.0;
         if (WANT_K) {
-          const T* aj = s_a + (le * 8 + j) * kAStride;
-          const T* ak = s_a + (le * 8 + k) * kAStride;
-          const T* w = s_w + le * 8;
 #pragma unroll
           for (int q = 0; q < 9; ++q) G[q] = T(0);
 #pragma unroll
           for (int gg = 0; gg < 4; ++gg) {
             const int g = 4 * h + gg;
-            const T wg = w[g];
-            const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
-            const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
-            G[0] = fma(u0, b0, G[0]); G[1] = fma(u0, b1, G[1]); G[2] = fma(u0, b2, G[2]);
-            G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
-            G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
+            const T* Ji = s_ji + (le * 8 + g) * JS;
+            const T* dj = s_dnr + (g * 8 + j) * 4;
+            T ji[10];
+#pragma unroll
+            for (int q = 0; q < 10; ++q) ji[q] = Ji[q];
+            const T dj0 = dj[0], dj1 = dj[1], dj2 = dj[2];
+            T u[3], b[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              u[a] = ji[9] * (dj0 * ji[a] + dj1 * ji[3 + a] + dj2 * ji[6 + a]);
+              b[a] = dk[gg][0] * ji[a] + dk[gg][1] * ji[3 + a] + dk[gg][2] * ji[6 + a];
+            }
+#pragma unroll
+            for (int p = 0; p < 3; ++p)
+#pragma unroll
+              for (int q = 0; q < 3; ++q) G[3 * p + q] = fma(u[p], b[q], G[3 * p + q]);
           }
 #pragma unroll
           for (int q = 0; q < 9; ++q) G[q] += __shfl_xor(G[q], 1, 64);
@@ -270,8 +328,7 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
       const T* G = s_acc + q9 * 9;
       T v = lh * G[3 * p + q] + mh * G[3 * q + p];
       if (p == q) v += mh * (G[0] + G[4] + G[8]);
-      const int32_t i = s_rows[lo];
-      const int64_t sm = m.sm_node[i];
+      const int64_t sm = s_sm[lo];
       const int64_t mloc = q9 - s_rowpre[lo];
       if (LAYOUT == FEM_LAYOUT_BSR) {
         k_vals[9 * (sm + mloc) + e9] = v;
@@ -288,7 +345,7 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
         const int mid = (lo + hi + 1) >> 1;
         if (s_rowpre[mid] <= t) lo = mid; else hi = mid - 1;
       }
-      m_blocks[(int64_t)m.sm_node[s_rows[lo]] + (t - s_rowpre[lo])] = s_accm[t];
+      m_blocks[(int64_t)s_sm[lo] + (t - s_rowpre[lo])] = s_accm[t];
     }
   }
 }
@@ -296,8 +353,8 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
 template <typename T, bool K, bool M, int LAYOUT>
 int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, double rho,
                void* k_vals, double* m_blocks, hipStream_t st) {
-  const int eg = m.max_group_elems, rg = m.max_group_rows, sg = m.max_group_slots;
-  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, K, M);
+  const int eg = m.max_group_elems, rg = m.max_group_rows, sg = m.max_group_slots, ig = m.max_group_incs;
+  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, ig, K, M);
   FEM_REQUIRE(L.total <= 160 * 1024, FEM_E_LIMIT,
               "row group needs %d B of LDS (> 160 KB): use smaller row groups", L.total);
   auto kern = hex8_assemble_kernel<T, K, M, LAYOUT>;
@@ -306,7 +363,7 @@ int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, dou
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
   hipLaunchKernelGGL(kern, dim3((unsigned)m.num_groups), dim3(kAsmThreads), L.total, st, m, young,
-                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg, sg);
+                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg, sg, ig);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -395,7 +452,7 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
                     m->grp_elem_ptr && m->grp_elems && m->ne_local,
                 FEM_E_INVALID, "mesh lacks coordinates, incidences or row groups");
     FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_rows >= 0 &&
-                    m->max_group_slots >= 0,
+                    m->max_group_slots >= 0 && m->max_group_incs >= 0,
                 FEM_E_INVALID, "bad group sizes");
   }
   return FEM_OK;
@@ -407,11 +464,13 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
 using namespace fem;
 
 extern "C" int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
-                                               int32_t max_group_slots, int k_dtype, int want_k,
-                                               int want_m) {
+                                               int32_t max_group_slots, int32_t max_group_incs,
+                                               int k_dtype, int want_k, int want_m) {
   if (k_dtype == FEM_DTYPE_F32)
-    return asm_lds_layout<float>(max_group_elems, max_group_rows, max_group_slots, want_k, want_m).total;
-  return asm_lds_layout<double>(max_group_elems, max_group_rows, max_group_slots, want_k, want_m).total;
+    return asm_lds_layout<float>(max_group_elems, max_group_rows, max_group_slots, max_group_incs, want_k,
+                                 want_m).total;
+  return asm_lds_layout<double>(max_group_elems, max_group_rows, max_group_slots, max_group_incs, want_k,
+                                want_m).total;
 }
 
 extern "C" int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_scalar,

@@ -1,0 +1,250 @@
+// Multi-right-hand-side kernels for block Lanczos (SOLVE_eigen.py:45-89):
+// block-Jacobi PCG on nb columns at once (the shift-invert solves
+// K^-1 (M u) of :65), with per-column scalars kept on the device.
+//
+// Blocks of vectors are node-interleaved with columns fastest:
+// X[(3*i + p) * nb + c].  nb is a power of two (the caller pads with zero
+// columns), so a block-level tree reduction over threads keeps the column
+// index (thread id mod nb) and sums every column in a fixed order.  As in the
+// single-vector PCG, each of kGrid blocks writes per-column partials and the
+// consuming kernel re-sums them in index order: no atomics, bit-reproducible.
+#include "common.hpp"
+
+namespace fem {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kGridB = 512;
+
+// Per-column sum over the block: s[tid] holds values whose column is tid % nb;
+// result for column c < nb ends in s[c].
+__device__ void col_reduce(double* s, int nb) {
+  for (int w = kT / 2; w >= nb; w >>= 1) {
+    __syncthreads();
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+  }
+  __syncthreads();
+}
+
+// out[c] = sum over blocks of part[b * nb + c] for every column c < nb, in
+// the same order in every block (out is shared memory).
+__device__ void col_partials(const double* __restrict__ part, int nb, double* s, double* out) {
+  // each thread sums blocks b = tid / nb, tid / nb + kT / nb, ... for column tid % nb
+  const int cc = threadIdx.x % nb;
+  double v = 0.0;
+  for (int b = threadIdx.x / nb; b < kGridB; b += kT / nb) v += part[b * nb + cc];
+  s[threadIdx.x] = v;
+  col_reduce(s, nb);
+  if ((int)threadIdx.x < nb) out[threadIdx.x] = s[threadIdx.x];
+  __syncthreads();
+}
+
+// Y = A X with per-column partial X.Y (A: BSR 3x3 f64).  Thread = (row i, c).
+__global__ void __launch_bounds__(kT)
+mspmm_dot_kernel(fem_mesh m, const double* __restrict__ vals, const double* __restrict__ X,
+                 double* __restrict__ Y, int nb, double* __restrict__ part, double* state) {
+  __shared__ double s[kT];
+  const int c = threadIdx.x % nb;
+  double d = 0.0;
+  const int64_t total = m.num_node * nb;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int64_t i = t / nb;
+    const int64_t sm = m.sm_node[i];
+    const int co = m.co_node[i];
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int b = 0; b < co; ++b) {
+      const double* v = vals + 9 * (sm + b);
+      const int64_t col = m.re_node[sm + b];
+      const double x0 = X[(3 * col) * nb + c], x1 = X[(3 * col + 1) * nb + c], x2 = X[(3 * col + 2) * nb + c];
+      a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+      a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+      a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    Y[(3 * i) * nb + c] = a0;
+    Y[(3 * i + 1) * nb + c] = a1;
+    Y[(3 * i + 2) * nb + c] = a2;
+    d += a0 * X[(3 * i) * nb + c] + a1 * X[(3 * i + 1) * nb + c] + a2 * X[(3 * i + 2) * nb + c];
+  }
+  if (part) {
+    s[threadIdx.x] = d;
+    col_reduce(s, nb);
+    if ((int)threadIdx.x < nb) part[blockIdx.x * nb + threadIdx.x] = s[threadIdx.x];
+    // move rz_next (row 1) into rz (row 0) for the update kernel
+    if (blockIdx.x == 0 && (int)threadIdx.x < nb) state[threadIdx.x] = state[nb + threadIdx.x];
+  }
+}
+
+// R = B - Q; Z = Dinv R; P = Z; partials rz, rr, bb per column.
+__global__ void __launch_bounds__(kT)
+mpcg_start_kernel(int64_t N, int nb, const double* __restrict__ B, const double* Q,
+                  const double* __restrict__ dinv, double* R, double* Z, double* P,
+                  double* __restrict__ part) {
+  __shared__ double s[kT];
+  const int c = threadIdx.x % nb;
+  double rz = 0, rr = 0, bb = 0;
+  const int64_t total = N * nb;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int64_t i = t / nb;
+    double r[3], z[3];
+    for (int a = 0; a < 3; ++a) r[a] = B[(3 * i + a) * nb + c] - Q[(3 * i + a) * nb + c];
+    const double* D = dinv + 9 * i;
+    for (int a = 0; a < 3; ++a) z[a] = D[3 * a] * r[0] + D[3 * a + 1] * r[1] + D[3 * a + 2] * r[2];
+    for (int a = 0; a < 3; ++a) {
+      R[(3 * i + a) * nb + c] = r[a];
+      Z[(3 * i + a) * nb + c] = z[a];
+      P[(3 * i + a) * nb + c] = z[a];
+      rz += r[a] * z[a];
+      rr += r[a] * r[a];
+      const double bv = B[(3 * i + a) * nb + c];
+      bb += bv * bv;
+    }
+  }
+  double vals3[3] = {rz, rr, bb};
+  for (int q = 0; q < 3; ++q) {
+    s[threadIdx.x] = vals3[q];
+    col_reduce(s, nb);
+    if ((int)threadIdx.x < nb) part[(q * kGridB + blockIdx.x) * nb + threadIdx.x] = s[threadIdx.x];
+    __syncthreads();
+  }
+}
+
+// state rows: 0 rz, 1 rz_next, 2 alpha, 3 pq, 4 rr, 5 bb, 6 beta, 7 spare.
+__global__ void mpcg_start_finalize(const double* __restrict__ part, int nb, double* state) {
+  __shared__ double s[kT];
+  __shared__ double o[3][64];
+  col_partials(part, nb, s, o[0]);
+  col_partials(part + kGridB * nb, nb, s, o[1]);
+  col_partials(part + 2 * kGridB * nb, nb, s, o[2]);
+  if ((int)threadIdx.x < nb) {
+    state[1 * nb + threadIdx.x] = o[0][threadIdx.x];
+    state[4 * nb + threadIdx.x] = o[1][threadIdx.x];
+    state[5 * nb + threadIdx.x] = o[2][threadIdx.x];
+  }
+}
+
+__global__ void __launch_bounds__(kT)
+mpcg_update_kernel(int64_t N, int nb, double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
+                   const double* __restrict__ P, const double* __restrict__ Q, const double* __restrict__ dinv,
+                   const double* __restrict__ part_pq, double* __restrict__ part_out, double* state) {
+  __shared__ double s[kT];
+  __shared__ double pq_s[64];
+  __shared__ double alpha_s[64];
+  col_partials(part_pq, nb, s, pq_s);
+  if ((int)threadIdx.x < nb) {
+    const int c = threadIdx.x;
+    const double pq = pq_s[c];
+    const double a = (pq != 0.0) ? state[c] / pq : 0.0;
+    alpha_s[c] = a;
+    if (blockIdx.x == 0) {
+      state[2 * nb + c] = a;
+      state[3 * nb + c] = pq;
+    }
+  }
+  __syncthreads();
+  const int c = threadIdx.x % nb;
+  const double a = alpha_s[c];
+  double rz = 0, rr = 0;
+  const int64_t total = N * nb;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+    const int64_t i = t / nb;
+    double r[3], z[3];
+    for (int q = 0; q < 3; ++q) {
+      const int64_t o = (3 * i + q) * nb + c;
+      X[o] += a * P[o];
+      r[q] = R[o] - a * Q[o];
+      R[o] = r[q];
+    }
+    const double* D = dinv + 9 * i;
+    for (int q = 0; q < 3; ++q) {
+      z[q] = D[3 * q] * r[0] + D[3 * q + 1] * r[1] + D[3 * q + 2] * r[2];
+      Z[(3 * i + q) * nb + c] = z[q];
+      rz += r[q] * z[q];
+      rr += r[q] * r[q];
+    }
+  }
+  s[threadIdx.x] = rz;
+  col_reduce(s, nb);
+  if ((int)threadIdx.x < nb) part_out[blockIdx.x * nb + threadIdx.x] = s[threadIdx.x];
+  __syncthreads();
+  s[threadIdx.x] = rr;
+  col_reduce(s, nb);
+  if ((int)threadIdx.x < nb) part_out[(kGridB + blockIdx.x) * nb + threadIdx.x] = s[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kT)
+mpcg_direction_kernel(int64_t N, int nb, const double* __restrict__ Z, double* __restrict__ P,
+                      const double* __restrict__ part, double* state) {
+  __shared__ double s[kT];
+  __shared__ double rz_s[64];
+  __shared__ double rr_s[64];
+  __shared__ double beta_s[64];
+  col_partials(part, nb, s, rz_s);
+  if (blockIdx.x == 0) col_partials(part + kGridB * nb, nb, s, rr_s);
+  if ((int)threadIdx.x < nb) {
+    const int c = threadIdx.x;
+    const double rz = state[c];
+    const double b = (rz != 0.0) ? rz_s[c] / rz : 0.0;
+    beta_s[c] = b;
+    if (blockIdx.x == 0) {
+      state[1 * nb + c] = rz_s[c];
+      state[4 * nb + c] = rr_s[c];
+      state[6 * nb + c] = b;
+    }
+  }
+  __syncthreads();
+  const int c = threadIdx.x % nb;
+  const double b = beta_s[c];
+  const int64_t total = 3 * N * nb;
+  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT)
+    P[t] = Z[t] + b * P[t];
+}
+
+int check_nb(int64_t nb) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, FEM_E_INVALID,
+              "nrhs must be a power of two in [1, 64] (pad with zero columns), got %lld", (long long)nb);
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" int64_t fem_mpcg_work_bytes(int64_t nrhs) { return 8 * (int64_t)kGridB * nrhs * 6 + 256; }
+
+extern "C" int fem_mpcg_start(const fem_mesh* mesh, const double* vals, const double* dinv, int64_t nrhs,
+                              const double* B, double* X, double* R, double* Z, double* P, double* state,
+                              void* work, int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(mesh && vals && dinv && B && X && R && Z && P && state && work, FEM_E_INVALID, "null pointer");
+  int rc = check_nb(nrhs);
+  if (rc) return rc;
+  FEM_REQUIRE(work_bytes >= fem_mpcg_work_bytes(nrhs), FEM_E_INVALID, "work too small");
+  if (mesh->num_node == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  const int nb = (int)nrhs;
+  mspmm_dot_kernel<<<kGridB, kT, 0, st>>>(*mesh, vals, X, Z, nb, nullptr, nullptr);
+  mpcg_start_kernel<<<kGridB, kT, 0, st>>>(mesh->num_node, nb, B, Z, dinv, R, Z, P, part);
+  mpcg_start_finalize<<<1, kT, 0, st>>>(part, nb, state);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const double* dinv, int64_t nrhs,
+                                double* X, double* R, double* Z, double* P, double* Q, double* state,
+                                void* work, int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(mesh && vals && dinv && X && R && Z && P && Q && state && work, FEM_E_INVALID, "null pointer");
+  int rc = check_nb(nrhs);
+  if (rc) return rc;
+  FEM_REQUIRE(work_bytes >= fem_mpcg_work_bytes(nrhs), FEM_E_INVALID, "work too small");
+  if (mesh->num_node == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  const int nb = (int)nrhs;
+  const int64_t N = mesh->num_node;
+  mspmm_dot_kernel<<<kGridB, kT, 0, st>>>(*mesh, vals, P, Q, nb, part, state);
+  mpcg_update_kernel<<<kGridB, kT, 0, st>>>(N, nb, X, R, Z, P, Q, dinv, part, part + kGridB * nb, state);
+  mpcg_direction_kernel<<<kGridB, kT, 0, st>>>(N, nb, Z, P, part + kGridB * nb, state);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}

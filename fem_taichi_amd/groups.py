@@ -85,8 +85,10 @@ def build_row_groups(N, E, ne_ptr, ne_list, gid, max_elems, co_node=None):
     grp_elems = (uniq % max(E, 1)).astype(np.int32)
     ne_local = (np.searchsorted(uniq, keys) - grp_elem_ptr[gid[inc_node]]).astype(np.int32)
     slots = np.bincount(gid, weights=co_node, minlength=G) if (co_node is not None and G) else np.zeros(1)
+    incs = np.bincount(gid, weights=np.diff(ne_ptr), minlength=G) if G else np.zeros(1)
     return dict(grp_row_ptr=grp_row_ptr, grp_rows=order.astype(np.int32), grp_elem_ptr=grp_elem_ptr,
                 grp_elems=grp_elems, ne_local=ne_local, num_groups=G,
                 max_group_elems=int(counts.max()) if G else 0,
                 max_group_rows=int(np.diff(grp_row_ptr).max()) if G else 0,
-                max_group_slots=int(slots.max()) if G else 0)
+                max_group_slots=int(slots.max()) if G else 0,
+                max_group_incs=int(incs.max()) if G else 0)

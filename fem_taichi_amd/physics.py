@@ -48,9 +48,10 @@ class DeviceMesh:
         m.max_group_elems = int(groups["max_group_elems"])
         m.max_group_rows = int(groups["max_group_rows"])
         m.max_group_slots = int(groups["max_group_slots"])
+        m.max_group_incs = int(groups["max_group_incs"])
         self.c = m
         self.groups_info = {k: groups[k] for k in ("num_groups", "max_group_elems", "max_group_rows",
-                                                   "max_group_slots")}
+                                                   "max_group_slots", "max_group_incs")}
 
     @property
     def ref(self):
@@ -134,7 +135,8 @@ class Physics:
         groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, MAX_GROUP_ELEMS,
                                   co_node=self.CoNode)
         lds = _lib.lib().fem_hex8_assemble_lds_bytes(groups["max_group_elems"], groups["max_group_rows"],
-                                                      groups["max_group_slots"], _lib.DTYPE_F64, 1, 1)
+                                                      groups["max_group_slots"], groups["max_group_incs"],
+                                                      _lib.DTYPE_F64, 1, 1)
         if lds > 160 * 1024:
             raise ValueError(f"row groups need {lds} B of LDS; use a smaller brick")
         self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)

@@ -75,7 +75,7 @@ typedef struct fem_mesh {
   int32_t max_group_elems;     /* max over groups of the element count */
   int32_t max_group_rows;      /* max over groups of the row count     */
   int32_t max_group_slots;     /* max over groups of sum CoNode of its rows */
-  int32_t pad0;
+  int32_t max_group_incs;      /* max over groups of the rows' incidence count */
   const int32_t* grp_row_ptr;  /* G + 1 */
   const int32_t* grp_rows;     /* N, rows in group order */
   const int32_t* grp_elem_ptr; /* G + 1 */
@@ -130,7 +130,8 @@ int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_sc
 /* LDS bytes one work-group of fem_hex8_assemble uses for a row group of the
  * given sizes (to check a grouping fits the 160 KB of a CU before launch). */
 int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
-                                    int32_t max_group_slots, int k_dtype, int want_k, int want_m);
+                                    int32_t max_group_slots, int32_t max_group_incs, int k_dtype,
+                                    int want_k, int want_m);
 
 /* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and
  *      ModifyMatrixK FEM_model_bsr.py:248-259) ------------------------------
@@ -207,6 +208,19 @@ int fem_pcg_iterate(const fem_mesh* mesh, const void* vals, const void* dinv, in
 int fem_pcg_replace_residual(const fem_mesh* mesh, const void* dinv, int dtype, const void* b,
                              const void* y, void* r, void* z, void* p_restart, double* state,
                              void* work, int64_t work_bytes, void* stream);
+
+/* ---- multi-RHS block-Jacobi PCG: the shift-invert solves K^-1 (M u) of
+ *      block Lanczos (SOLVE_eigen.py:65, there a Pardiso solve) ------------
+ * Blocks are node-interleaved with columns fastest: X[(3*i+p)*nrhs + c];
+ * nrhs is a power of two <= 64 (pad with zero columns).  state is 8 x nrhs
+ * doubles (rows: rz, rz_next, alpha, p.q, r.r, b.b, beta, spare). */
+int64_t fem_mpcg_work_bytes(int64_t nrhs);
+int fem_mpcg_start(const fem_mesh* mesh, const double* vals, const double* dinv, int64_t nrhs,
+                   const double* B, double* X, double* R, double* Z, double* P, double* state,
+                   void* work, int64_t work_bytes, void* stream);
+int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const double* dinv, int64_t nrhs,
+                     double* X, double* R, double* Z, double* P, double* Q, double* state,
+                     void* work, int64_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }
