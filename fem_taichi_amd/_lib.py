@@ -87,6 +87,14 @@ SIGNATURES = {
     "fem_mpcg_work_bytes": (_I64, [_I64]),
     "fem_mpcg_start": (_I32, [_MP, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
     "fem_mpcg_iterate": (_I32, [_MP, _P, _P, _I64, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+    "fem_meshconv_fwd_bf16": (_I32, [_P, _I32, _P, _P, _I32, _I32, _I32, _I32, _P, _P, _P, _I32, _P]),
+    "fem_meshconv_flip_bf16": (_I32, [_P, _I32, _I32, _P, _I32, _I32, _P]),
+    "fem_meshconv_wgrad_work_bytes": (_I64, [_I32, _I32, _I32, _I32, _I32, _I32]),
+    "fem_meshconv_wgrad_bf16": (_I32, [_P, _I32, _P, _I32, _I32, _I32, _I32, _I32, _P, _P, _I64, _P]),
+    "fem_colsum_work_bytes": (_I64, [_I64, _I32, _I32]),
+    "fem_colsum": (_I32, [_P, _I32, _I64, _I32, _I32, _P, _P, _I64, _P]),
+    "fem_relu_bwd_bf16": (_I32, [_P, _P, _I64, _P, _P]),
+    "fem_to_bf16": (_I32, [_P, _I64, _I32, _I32, _P, _P]),
     "fem_pcg_replace_residual": (_I32, [_MP, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
 }
 

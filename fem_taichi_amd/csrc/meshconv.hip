@@ -1,0 +1,302 @@
+// 3x3x3 mesh convolution on the structured node grid (the learned
+// preconditioner's layers), BF16 operands on MFMA with FP32 accumulation.
+//
+// Layout: activations are channels-last, x[p * C + c], p = ix + nx*(iy +
+// ny*iz) (the node index of a structured grid), C padded to a multiple of 32
+// (inputs) / 16 (outputs); weights w[tap][co][ci] with ci contiguous, tap =
+// (dz+1)*9 + (dy+1)*3 + (dx+1); out-of-grid neighbours read as zero.
+//
+// Forward (implicit GEMM): a work-group computes 64 pixels x Cout; wave w
+// owns pixels [16w, 16w+16) and Cout/16 accumulator tiles of
+// v_mfma_f32_16x16x32_bf16; the K loop runs over (tap, 32-channel chunk).
+// Operand layout of that instruction: lane l supplies A[l%16][8*(l/16)+i]
+// and B[8*(l/16)+i][l%16], i < 8, and receives D[4*(l/16)+i][l%16], i < 4.
+// Backward data is the forward kernel on flipped, channel-transposed
+// weights.  The weight gradient is a GEMM over pixels: each work-group owns
+// (tap, pixel chunk), stages 32-pixel tiles of dy and of the shifted input in
+// LDS transposed (so each lane's 8 pixels are contiguous), and writes a
+// partial; partials are summed over chunks in index order (deterministic).
+#include "common.hpp"
+
+namespace fem {
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kCT = 256;
+constexpr int kTileP = 64;
+
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) z[i] = (bf16)0.0f;
+  return z;
+}
+
+template <int NT>
+__global__ void __launch_bounds__(kCT)
+conv3_fwd_kernel(const bf16* __restrict__ x, int cin, const bf16* __restrict__ w,
+                 const float* __restrict__ bias, int cout, int nx, int ny, int nz,
+                 bf16* __restrict__ y_act, bf16* __restrict__ y_pre, float* __restrict__ y_f32, int relu) {
+  const int64_t P = (int64_t)nx * ny * nz;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int64_t pbase = (int64_t)blockIdx.x * kTileP + wave * 16;
+  const int64_t p = pbase + m;
+  const bool pin = p < P;
+  const int px = pin ? (int)(p % nx) : 0;
+  const int py = pin ? (int)((p / nx) % ny) : 0;
+  const int pz = pin ? (int)(p / ((int64_t)nx * ny)) : 0;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 27; ++tap) {
+    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+    const int qx = px + dx, qy = py + dy, qz = pz + dz;
+    const bool valid = pin && qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz;
+    const int64_t q = ((int64_t)qz * ny + qy) * nx + qx;
+    for (int c0 = 0; c0 < cin; c0 += 32) {
+      const bf16x8 A = valid ? *reinterpret_cast<const bf16x8*>(x + q * cin + c0 + kq * 8) : zero8();
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x8 B =
+            *reinterpret_cast<const bf16x8*>(w + ((int64_t)(tap * cout + t * 16 + m)) * cin + c0 + kq * 8);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int ch = t * 16 + m;
+    const float b = bias ? bias[ch] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int64_t pp = pbase + kq * 4 + i;
+      if (pp >= P) continue;
+      const float v = acc[t][i] + b;
+      if (y_pre) y_pre[pp * cout + ch] = (bf16)v;
+      if (y_f32) y_f32[pp * cout + ch] = v;
+      if (y_act) y_act[pp * cout + ch] = (bf16)((relu && v < 0.f) ? 0.f : v);
+    }
+  }
+}
+
+// wt[tap][ci][co] = w[26 - tap][co][ci]   (backward-data weights)
+__global__ void conv3_flip_kernel(const bf16* __restrict__ w, int cout, int cin, bf16* __restrict__ wt,
+                                  int cout_t, int cin_t) {
+  // wt has "cout" = cin (padded to cout_t, a multiple of 16) and "cin" = cout
+  // (padded to cin_t, a multiple of 32)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = 27LL * cout_t * cin_t;
+  if (t >= total) return;
+  const int tap = (int)(t / ((int64_t)cout_t * cin_t));
+  const int r = (int)(t % ((int64_t)cout_t * cin_t));
+  const int ci = r / cin_t, co = r % cin_t;   // wt[tap][ci][co]
+  bf16 v = (bf16)0.0f;
+  if (ci < cin && co < cout) v = w[((int64_t)(26 - tap) * cout + co) * cin + ci];
+  wt[t] = v;
+}
+
+// dW partial for one (tap, pixel chunk): dw[co][ci] = sum_p dy[p][co] x[p+off][ci]
+template <int NT>
+__global__ void __launch_bounds__(kCT)
+conv3_wgrad_kernel(const bf16* __restrict__ x, int cin, const bf16* __restrict__ dy, int cout, int nx, int ny,
+                   int nz, int chunk, float* __restrict__ part) {
+  constexpr int LD = 40;   // LDS row stride in bf16 (32 pixels + 8 pad)
+  __shared__ __align__(16) bf16 sdy[64 * LD];
+  __shared__ __align__(16) bf16 sx[64 * LD];
+  const int64_t P = (int64_t)nx * ny * nz;
+  const int tap = blockIdx.x;
+  const int dz = tap / 9 - 1, dyo = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+  const int64_t p0 = (int64_t)blockIdx.y * chunk;
+  const int64_t p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int mt = wave;   // co tile of this wave
+  const bool active = mt * 16 < cout;
+  f32x4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int64_t pb = p0; pb < p1; pb += 32) {
+    // stage dy[32 px][cout] and x[32 px shifted][cin], transposed
+    for (int e = threadIdx.x; e < 32 * 64; e += kCT) {
+      const int pl = e / 64, c = e % 64;
+      const int64_t pp = pb + pl;
+      bf16 vdy = (bf16)0.0f, vx = (bf16)0.0f;
+      if (pp < p1) {
+        if (c < cout) vdy = dy[pp * cout + c];
+        if (c < cin) {
+          const int px = (int)(pp % nx), py = (int)((pp / nx) % ny), pz = (int)(pp / ((int64_t)nx * ny));
+          const int qx = px + dx, qy = py + dyo, qz = pz + dz;
+          if (qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz)
+            vx = x[(((int64_t)qz * ny + qy) * nx + qx) * cin + c];
+        }
+      }
+      sdy[c * LD + pl] = vdy;
+      sx[c * LD + pl] = vx;
+    }
+    __syncthreads();
+    if (active) {
+      const bf16x8 A = *reinterpret_cast<const bf16x8*>(&sdy[(mt * 16 + m) * LD + kq * 8]);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        if (t * 16 < cin) {
+          const bf16x8 B = *reinterpret_cast<const bf16x8*>(&sx[(t * 16 + m) * LD + kq * 8]);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!active) return;
+  float* dst = part + ((int64_t)blockIdx.y * 27 + tap) * cout * cin;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (t * 16 >= cin) continue;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) dst[(mt * 16 + kq * 4 + i) * cin + t * 16 + m] = acc[t][i];
+  }
+}
+
+// dw[o] = sum over chunks of part[chunk][o] in chunk order.
+__global__ void sum_chunks_kernel(const float* __restrict__ part, int64_t per, int nchunk, float* __restrict__ out) {
+  const int64_t o = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= per) return;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * per + o];
+  out[o] = s;
+}
+
+// Per-channel column sums of a (P x C) f32/bf16 matrix: partial per block.
+template <typename T>
+__global__ void col_sum_partial_kernel(const T* __restrict__ a, int64_t P, int C, int chunk, float* __restrict__ part) {
+  const int64_t p0 = (int64_t)blockIdx.x * chunk;
+  const int64_t p1 = (p0 + chunk < P) ? p0 + chunk : P;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    float s = 0.f;
+    for (int64_t p = p0; p < p1; ++p) s += (float)a[p * C + c];
+    part[(int64_t)blockIdx.x * C + c] = s;
+  }
+}
+
+// dz = da * (z > 0)
+__global__ void relu_bwd_kernel(const bf16* __restrict__ da, const bf16* __restrict__ z, int64_t n,
+                                bf16* __restrict__ dz) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  dz[t] = ((float)z[t] > 0.f) ? da[t] : (bf16)0.0f;
+}
+
+// f32 (P x cs, cs source channels) -> bf16 (P x cd), zero-padded channels
+__global__ void to_bf16_kernel(const float* __restrict__ a, int64_t P, int cs, int cd, bf16* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P * cd) return;
+  const int64_t p = t / cd;
+  const int c = (int)(t % cd);
+  out[t] = (bf16)((c < cs) ? a[p * cs + c] : 0.f);
+}
+
+int conv_check(int cin, int cout, int nx, int ny, int nz) {
+  FEM_REQUIRE(cin > 0 && cin % 32 == 0 && cin <= 64, FEM_E_INVALID, "cin must be 32 or 64 (padded), got %d", cin);
+  FEM_REQUIRE(cout > 0 && cout % 16 == 0 && cout <= 64, FEM_E_INVALID, "cout must be a multiple of 16 <= 64");
+  FEM_REQUIRE(nx > 0 && ny > 0 && nz > 0, FEM_E_INVALID, "bad grid");
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" int fem_meshconv_fwd_bf16(const void* x, int cin, const void* w, const float* bias, int cout, int nx,
+                                     int ny, int nz, void* y_act, void* y_pre, float* y_f32, int relu,
+                                     void* stream) {
+  int rc = conv_check(cin, cout, nx, ny, nz);
+  if (rc) return rc;
+  FEM_REQUIRE(x && w && (y_act || y_pre || y_f32), FEM_E_INVALID, "null pointer");
+  const int64_t P = (int64_t)nx * ny * nz;
+  const dim3 grid((unsigned)ceil_div(P, kTileP));
+  hipStream_t st = as_stream(stream);
+  const bf16* xb = (const bf16*)x;
+  const bf16* wb = (const bf16*)w;
+  switch (cout / 16) {
+    case 1: conv3_fwd_kernel<1><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
+    case 2: conv3_fwd_kernel<2><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
+    case 3: conv3_fwd_kernel<3><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
+    default: conv3_fwd_kernel<4><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
+  }
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_meshconv_flip_bf16(const void* w, int cout, int cin, void* wt, int cout_t, int cin_t,
+                                      void* stream) {
+  FEM_REQUIRE(w && wt && cout_t % 16 == 0 && cin_t % 32 == 0 && cout_t >= cin && cin_t >= cout, FEM_E_INVALID,
+              "bad flip arguments");
+  const int64_t total = 27LL * cout_t * cin_t;
+  conv3_flip_kernel<<<ceil_div(total, 256), 256, 0, as_stream(stream)>>>((const bf16*)w, cout, cin, (bf16*)wt,
+                                                                        cout_t, cin_t);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int64_t fem_meshconv_wgrad_work_bytes(int cin, int cout, int nx, int ny, int nz, int chunk) {
+  const int64_t P = (int64_t)nx * ny * nz;
+  return 4LL * ceil_div(P, chunk) * 27 * cin * cout + 256;
+}
+
+extern "C" int fem_meshconv_wgrad_bf16(const void* x, int cin, const void* dy, int cout, int nx, int ny, int nz,
+                                       int chunk, float* dw, void* work, int64_t work_bytes, void* stream) {
+  int rc = conv_check(cin, cout, nx, ny, nz);
+  if (rc) return rc;
+  FEM_REQUIRE(chunk > 0 && chunk % 32 == 0, FEM_E_INVALID, "chunk must be a positive multiple of 32");
+  FEM_REQUIRE(x && dy && dw && work, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(work_bytes >= fem_meshconv_wgrad_work_bytes(cin, cout, nx, ny, nz, chunk), FEM_E_INVALID,
+              "work too small");
+  const int64_t P = (int64_t)nx * ny * nz;
+  const int nchunk = (int)ceil_div(P, chunk);
+  hipStream_t st = as_stream(stream);
+  float* part = reinterpret_cast<float*>(work);
+  const dim3 grid(27, nchunk);
+  conv3_wgrad_kernel<4><<<grid, kCT, 0, st>>>((const bf16*)x, cin, (const bf16*)dy, cout, nx, ny, nz, chunk, part);
+  const int64_t per = 27LL * cout * cin;
+  sum_chunks_kernel<<<ceil_div(per, 256), 256, 0, st>>>(part, per, nchunk, dw);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int64_t fem_colsum_work_bytes(int64_t P, int C, int chunk) { return 4LL * ceil_div(P, chunk) * C + 256; }
+
+extern "C" int fem_colsum(const void* a, int is_bf16, int64_t P, int C, int chunk, float* out, void* work,
+                          int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(a && out && work && P > 0 && C > 0 && chunk > 0, FEM_E_INVALID, "bad arguments");
+  FEM_REQUIRE(work_bytes >= fem_colsum_work_bytes(P, C, chunk), FEM_E_INVALID, "work too small");
+  const int nchunk = (int)ceil_div(P, chunk);
+  hipStream_t st = as_stream(stream);
+  float* part = reinterpret_cast<float*>(work);
+  if (is_bf16)
+    col_sum_partial_kernel<bf16><<<nchunk, 64, 0, st>>>((const bf16*)a, P, C, chunk, part);
+  else
+    col_sum_partial_kernel<float><<<nchunk, 64, 0, st>>>((const float*)a, P, C, chunk, part);
+  sum_chunks_kernel<<<ceil_div(C, 256), 256, 0, st>>>(part, C, nchunk, out);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_relu_bwd_bf16(const void* da, const void* z, int64_t n, void* dz, void* stream) {
+  FEM_REQUIRE(da && z && dz && n >= 0, FEM_E_INVALID, "bad arguments");
+  if (n == 0) return FEM_OK;
+  relu_bwd_kernel<<<ceil_div(n, 256), 256, 0, as_stream(stream)>>>((const bf16*)da, (const bf16*)z, n, (bf16*)dz);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_to_bf16(const float* a, int64_t P, int cs, int cd, void* out, void* stream) {
+  FEM_REQUIRE(a && out && P >= 0 && cs > 0 && cd >= cs, FEM_E_INVALID, "bad arguments");
+  if (P == 0) return FEM_OK;
+  to_bf16_kernel<<<ceil_div(P * cd, 256), 256, 0, as_stream(stream)>>>(a, P, cs, cd, (bf16*)out);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}

@@ -222,6 +222,36 @@ int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const double* din
                      double* X, double* R, double* Z, double* P, double* Q, double* state,
                      void* work, int64_t work_bytes, void* stream);
 
+/* ---- 3x3x3 mesh convolution on the structured node grid (the learned
+ *      preconditioner; no counterpart in the reference) ---------------------
+ * BF16 operands on MFMA, FP32 accumulation.  Activations channels-last
+ * x[p*C + c], p = ix + nx*(iy + ny*iz); cin a multiple of 32 (<= 64), cout a
+ * multiple of 16 (<= 64); weights w[tap][co][ci], tap = (dz+1)*9 +
+ * (dy+1)*3 + (dx+1); zero padding outside the grid.  Pointers to bf16 data
+ * are void*. */
+/* y = conv(x) + bias; writes any of y_act (bf16, ReLU when relu != 0),
+ * y_pre (bf16, before ReLU), y_f32 (f32, before ReLU). */
+int fem_meshconv_fwd_bf16(const void* x, int cin, const void* w, const float* bias, int cout, int nx,
+                          int ny, int nz, void* y_act, void* y_pre, float* y_f32, int relu,
+                          void* stream);
+/* wt[tap][ci][co] = w[26-tap][co][ci], padded to (cout_t, cin_t): the
+ * weights of the backward-data convolution. */
+int fem_meshconv_flip_bf16(const void* w, int cout, int cin, void* wt, int cout_t, int cin_t,
+                           void* stream);
+/* dw[tap][co][ci] (f32) = sum_p dy[p][co] x[p + off(tap)][ci], split over
+ * pixel chunks and reduced in chunk order. */
+int64_t fem_meshconv_wgrad_work_bytes(int cin, int cout, int nx, int ny, int nz, int chunk);
+int fem_meshconv_wgrad_bf16(const void* x, int cin, const void* dy, int cout, int nx, int ny, int nz,
+                            int chunk, float* dw, void* work, int64_t work_bytes, void* stream);
+/* out[c] = sum_p a[p*C + c] (a f32 or bf16), deterministic. */
+int64_t fem_colsum_work_bytes(int64_t P, int C, int chunk);
+int fem_colsum(const void* a, int is_bf16, int64_t P, int C, int chunk, float* out, void* work,
+               int64_t work_bytes, void* stream);
+/* dz = da * (z > 0), bf16. */
+int fem_relu_bwd_bf16(const void* da, const void* z, int64_t n, void* dz, void* stream);
+/* out (P x cd, bf16) = a (P x cs, f32), channels cs..cd-1 zero. */
+int fem_to_bf16(const float* a, int64_t P, int cs, int cd, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
