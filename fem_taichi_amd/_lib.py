@@ -46,10 +46,11 @@ class FemMesh(ctypes.Structure):
         ("ne_ptr", ctypes.c_void_p), ("ne_list", ctypes.c_void_p),
         ("num_groups", ctypes.c_int64), ("max_group_elems", ctypes.c_int32),
         ("max_group_rows", ctypes.c_int32), ("max_group_slots", ctypes.c_int32),
-        ("max_group_incs", ctypes.c_int32),
+        ("max_group_tasks", ctypes.c_int32),
         ("grp_row_ptr", ctypes.c_void_p), ("grp_rows", ctypes.c_void_p),
         ("grp_elem_ptr", ctypes.c_void_p), ("grp_elems", ctypes.c_void_p),
-        ("ne_local", ctypes.c_void_p),
+        ("grp_enodes", ctypes.c_void_p),
+        ("grp_task_ptr", ctypes.c_void_p), ("tasks", ctypes.c_void_p),
     ]
 
 
@@ -71,7 +72,7 @@ SIGNATURES = {
     "fem_csr_index_M": (_I32, [_MP, _P, _P, _P]),
     "fem_csr_values_M": (_I32, [_MP, _P, _P, _P]),
     "fem_hex8_assemble": (_I32, [_MP, _P, _D, _D, _D, _P, _I32, _I32, _P, _P]),
-    "fem_hex8_assemble_lds_bytes": (_I64, [ctypes.c_int32] * 4 + [_I32, _I32, _I32]),
+    "fem_hex8_assemble_lds_bytes": (_I64, [ctypes.c_int32] * 2 + [_I32, _I32, _I32]),
     "fem_apply_penalty": (_I32, [_MP, _P, _P, _I64, _P, _I32, _I32, _D, _P]),
     "fem_body_force_work_bytes": (_I64, [_I64]),
     "fem_body_force": (_I32, [_MP, _D, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _I64, _P]),

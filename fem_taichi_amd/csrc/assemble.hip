@@ -5,347 +5,274 @@
 // computes an E x 8 x 8 x 3 x 3 element intermediate and scatters it with
 // atomic += after a linear slot search.  Here one work-group owns a group of
 // node rows (a spatially compact set built once from the pattern):
-//   phase 0: the group's row list, neighbour lists and incidence lists are
-//            staged in LDS (so the rounds below touch no global memory);
 //   phase 1: one thread per (element, Gauss point) of the elements touching
-//            the rows computes J, stores J^-1 and w = E_e*detJ in LDS;
-//   phase 2: 16 lanes own one row: lane = (local column node k, half h of
-//            the Gauss points).  Round t processes the row's t-th incidence
-//            (element e, local row node j) in ascending element order: the
-//            lane forms the physical gradients a_gj = DNr_g[j] J_g^-1 and
-//            a_gk, accumulates the 3x3 Gram block sum_g w_g a_gj (x) a_gk over
-//            its four Gauss points, combines the two halves with a lane
-//            shuffle and adds the block to the row's LDS accumulator at the
-//            slot of column node k.  Within a round the 8 lanes of a row hit
-//            8 distinct slots; rounds are ordered, so every block is a sum in
-//            ascending element order -- bit-identical run to run, no atomics;
-//   phase 3: every block of the group's rows is written to HBM once,
-//            coalesced, as K_block = lhat*G + mhat*G^T + mhat*tr(G)*I.
-// Phase 3's closed form is B_j^T D B_k with D = E*[lhat 1 1^T + 2 mhat I on
-// the normal block, mhat I on the shear block] (lhat = nu/((1+nu)(1-2nu)),
+//            the rows computes J and stages in LDS the physical gradients
+//            a[e][k][g] = DNr_g[k] J_g^-1 and w[e][g] = E_e * detJ_g;
+//   phase 2: one lane per task.  A task holds at most two element
+//            contributions of one output block (row i, column node c): the
+//            3x3 Gram block sum_g w_g a_gj (x) a_gk with j, k the local
+//            indices of i and c in the element.  The T tasks of a block are
+//            T adjacent, T-aligned lanes and are summed by a fixed butterfly
+//            of lane shuffles, so every block is a sum in a fixed order --
+//            bit-identical run to run, no atomics, no LDS accumulator.  The
+//            head lane writes K_block = lhat*G + mhat*G^T + mhat*tr(G)*I.
+//            Blocks with a single contribution are paired two per lane.
+// The closed form is B_j^T D B_k with D = E*[lhat 1 1^T + 2 mhat I on the
+// normal block, mhat I on the shear block] (lhat = nu/((1+nu)(1-2nu)),
 // mhat = 1/(2(1+nu))): K is linear in G and E is folded into w, so the
 // combination is applied once per block after the sum over elements.
+// The task lists are symbolic metadata built once per mesh (groups.py).
 #include "common.hpp"
 
 namespace fem {
 namespace {
 
-constexpr int kAsmThreads = 256;
-constexpr int kLanesPerRow = 16;
-constexpr int kRowsPerPass = kAsmThreads / kLanesPerRow;
-
-template <typename T>
-struct JStride {   // per (element, Gauss point): J^-1 (9) + w (1), padded for 16-B loads
-  static constexpr int value = (sizeof(T) == 8) ? 10 : 12;
-};
+constexpr int kAsmThreads = 512;
+constexpr int kAStride = 25;     // (e, k) stride of a[], elements of T: 24 used + 1 pad (LDS banks)
+constexpr int kNone = 0x3FFF;
 
 struct AsmLds {
-  int ji_off, acc_off, wm_off, accm_off, n_off, dnr_off, nodes_off, dup_off, rows_off, rowpre_off, sm_off,
-      incpre_off, re_off, inc_off, slot_off, total;
+  int a_off, w_off, wm_off, n_off, sm_off, co_off, total;
 };
 
 __host__ __device__ inline int align16(int o) { return (o + 15) & ~15; }
 
+// Work-groups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Give each XCD one contiguous run of the (spatially ordered) row
+// groups so that the coordinates neighbouring groups share stay in its L2.
+__device__ __forceinline__ int xcd_chunked(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 template <typename T>
-__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, int sg, int ig, bool want_k, bool want_m) {
+__host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bool want_m) {
   AsmLds L;
   int o = 0;
-  L.ji_off = o;
-  o += eg * 8 * JStride<T>::value * (int)sizeof(T);
+  L.a_off = o;
+  if (want_k) o += eg * 8 * kAStride * (int)sizeof(T);
   o = align16(o);
-  L.acc_off = o;
-  if (want_k) o += sg * 9 * (int)sizeof(T);
+  L.w_off = o;
+  if (want_k) o += eg * 8 * (int)sizeof(T);
   o = align16(o);
   L.wm_off = o;
   if (want_m) o += eg * 8 * (int)sizeof(double);
   o = align16(o);
-  L.accm_off = o;
-  if (want_m) o += sg * (int)sizeof(double);
-  o = align16(o);
   L.n_off = o;
   if (want_m) o += 64 * (int)sizeof(double);
-  L.dnr_off = o;
-  if (want_k) o += 8 * 8 * 4 * (int)sizeof(T);   // DNr[g][k][0..2] padded to 4
-  o = align16(o);
-  L.nodes_off = o;
-  o += eg * 8 * (int)sizeof(int32_t);
-  L.dup_off = o;
-  o += eg * (int)sizeof(int32_t);
-  L.rows_off = o;
-  o += rg * (int)sizeof(int32_t);
-  L.rowpre_off = o;
-  o += (rg + 1) * (int)sizeof(int32_t);
   L.sm_off = o;
   o += rg * (int)sizeof(int32_t);
-  L.incpre_off = o;
-  o += (rg + 1) * (int)sizeof(int32_t);
-  L.re_off = o;
-  o += sg * (int)sizeof(int32_t);
-  L.inc_off = o;
-  o += ig * (int)sizeof(int32_t);
-  L.slot_off = o;
-  o += ig * 8 * (int)sizeof(int16_t);
+  L.co_off = o;
+  o += rg * (int)sizeof(int32_t);
   L.total = align16(o);
   return L;
 }
 
+// dN_k/dxi at a Gauss point from its factors f_a = 1 + s_ka * sg_a * GS.
+__device__ __forceinline__ void dn_factored(int k, const double fp[3], const double fm[3], double dn[3]) {
+  const double f0 = node_sign(k, 0) > 0 ? fp[0] : fm[0];
+  const double f1 = node_sign(k, 1) > 0 ? fp[1] : fm[1];
+  const double f2 = node_sign(k, 2) > 0 ? fp[2] : fm[2];
+  dn[0] = 0.125 * node_sign(k, 0) * (f1 * f2);
+  dn[1] = 0.125 * node_sign(k, 1) * (f0 * f2);
+  dn[2] = 0.125 * node_sign(k, 2) * (f0 * f1);
+}
+
+template <typename T>
+__device__ __forceinline__ void gram(const T* __restrict__ s_a, const T* __restrict__ s_w, int c, T G[9]) {
+  // G += sum_g w_g a_gj (x) a_gk for contribution c (scaled by 0 when absent)
+  const bool has = c != kNone;
+  const int cc = has ? c : 0;
+  const int le = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
+  const T* aj = s_a + (le * 8 + j) * kAStride;
+  const T* ak = s_a + (le * 8 + k) * kAStride;
+  const T* w = s_w + le * 8;
+#pragma unroll 2
+  for (int g = 0; g < 8; ++g) {
+    const T wg = has ? w[g] : T(0);
+    const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
+    const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
+    G[0] = fma(u0, b0, G[0]); G[1] = fma(u0, b1, G[1]); G[2] = fma(u0, b2, G[2]);
+    G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
+    G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
+  }
+}
+
+__device__ __forceinline__ double mass_term(const double* __restrict__ s_wm, const double* __restrict__ s_N, int c) {
+  if (c == kNone) return 0.0;
+  const int le = c >> 6, j = (c >> 3) & 7, k = c & 7;
+  double ms = 0.0;
+#pragma unroll
+  for (int g = 0; g < 8; ++g) ms = fma(s_wm[le * 8 + g] * s_N[g * 8 + j], s_N[g * 8 + k], ms);
+  return ms;
+}
+
+template <typename T, int LAYOUT>
+__device__ __forceinline__ void store_block(T* __restrict__ k_vals, const int32_t* s_sm, const int32_t* s_co,
+                                            int out, const T G[9], T lh, T mh) {
+  const int r = out >> 8, m = out & 255;
+  const int64_t sm = s_sm[r];
+  const T tr = mh * (G[0] + G[4] + G[8]);
+  T v[9];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
+  if (LAYOUT == FEM_LAYOUT_BSR) {
+    T* dst = k_vals + 9 * (sm + m);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) dst[q] = v[q];
+  } else {
+    const int64_t co = s_co[r];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) k_vals[9 * sm + 3 * p * co + 3 * m + q] = v[3 * p + q];
+  }
+}
+
+// 4 waves per SIMD (<= 128 VGPRs): two work-groups of 512 threads per CU.
 template <typename T, bool WANT_K, bool WANT_M, int LAYOUT>
-__global__ void __launch_bounds__(kAsmThreads)
+__global__ void __launch_bounds__(kAsmThreads, 4)
 hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar,
                      double lhat, double mhat, double rho, T* __restrict__ k_vals,
-                     double* __restrict__ m_blocks, int eg, int rg, int sg, int ig) {
-  constexpr int JS = JStride<T>::value;
+                     double* __restrict__ m_blocks, int eg, int rg) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, ig, WANT_K, WANT_M);
-  T* s_ji = reinterpret_cast<T*>(smem + L.ji_off);
-  T* s_acc = reinterpret_cast<T*>(smem + L.acc_off);
+  const AsmLds L = asm_lds_layout<T>(eg, rg, WANT_K, WANT_M);
+  T* s_a = reinterpret_cast<T*>(smem + L.a_off);
+  T* s_w = reinterpret_cast<T*>(smem + L.w_off);
   double* s_wm = reinterpret_cast<double*>(smem + L.wm_off);
-  double* s_accm = reinterpret_cast<double*>(smem + L.accm_off);
   double* s_N = reinterpret_cast<double*>(smem + L.n_off);
-  T* s_dnr = reinterpret_cast<T*>(smem + L.dnr_off);
-  int32_t* s_nodes = reinterpret_cast<int32_t*>(smem + L.nodes_off);
-  int32_t* s_dup = reinterpret_cast<int32_t*>(smem + L.dup_off);
-  int32_t* s_rows = reinterpret_cast<int32_t*>(smem + L.rows_off);
-  int32_t* s_rowpre = reinterpret_cast<int32_t*>(smem + L.rowpre_off);
   int32_t* s_sm = reinterpret_cast<int32_t*>(smem + L.sm_off);
-  int32_t* s_incpre = reinterpret_cast<int32_t*>(smem + L.incpre_off);
-  int32_t* s_re = reinterpret_cast<int32_t*>(smem + L.re_off);
-  int32_t* s_inc = reinterpret_cast<int32_t*>(smem + L.inc_off);
-  int16_t* s_slot = reinterpret_cast<int16_t*>(smem + L.slot_off);
+  int32_t* s_co = reinterpret_cast<int32_t*>(smem + L.co_off);
 
-  const int grp = blockIdx.x;
+  const int grp = xcd_chunked(blockIdx.x, gridDim.x);
   const int e0 = m.grp_elem_ptr[grp];
   const int ne = m.grp_elem_ptr[grp + 1] - e0;
   const int r0 = m.grp_row_ptr[grp];
   const int nr = m.grp_row_ptr[grp + 1] - r0;
+  const int t0 = m.grp_task_ptr[grp];
+  const int nt = m.grp_task_ptr[grp + 1] - t0;
   const int tid = threadIdx.x;
 
-  // ---- phase 0a: per-row scalars and element node lists (independent loads)
-  for (int t = tid; t < ne * 8; t += kAsmThreads) {
-    const int e = m.grp_elems[e0 + (t >> 3)];
-    s_nodes[t] = m.elem[(int64_t)e * 8 + (t & 7)];
+  // Every load that depends only on the group is issued here, before any
+  // wait: this lane's first task, its element's node list and modulus, and
+  // (for the first nr lanes) the row pointers.
+  const int2 none_task = make_int2((int)0xFFFF0000u, kNone | (kNone << 14));
+  int2 task = tid < nt ? reinterpret_cast<const int2*>(m.tasks)[t0 + tid] : none_task;
+  const bool p1 = tid < ne * 8;            // ne <= 64 (checked on the host)
+  const int le = tid >> 3, g = tid & 7;
+  int4 na = make_int4(0, 0, 0, 0), nb = na;
+  double Ee = young_scalar;
+  if (p1) {
+    const int4* en = reinterpret_cast<const int4*>(m.grp_enodes + (int64_t)(e0 + le) * 8);
+    na = en[0];
+    nb = en[1];
+    if (young) Ee = young[m.grp_elems[e0 + le]];
   }
-  for (int r = tid; r < nr; r += kAsmThreads) {
-    const int32_t i = m.grp_rows[r0 + r];
-    s_rows[r] = i;
-    s_rowpre[r + 1] = m.co_node[i];                    // counts; scanned below
-    s_sm[r] = m.sm_node[i];
-    s_incpre[r + 1] = m.ne_ptr[i + 1] - m.ne_ptr[i];
+  if (tid < nr) {
+    const int32_t i = m.grp_rows[r0 + tid];
+    s_sm[tid] = m.sm_node[i];
+    s_co[tid] = m.co_node[i];
   }
-  if (WANT_M && tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);          // N[g*8 + k]
-  if (WANT_K && tid < 192) {                                              // DNr[g][k][d]
-    const int g = tid / 24, k = (tid / 3) % 8, d = tid % 3;
-    s_dnr[(g * 8 + k) * 4 + d] = (T)dnr(g, k, d);
-  }
-  __syncthreads();
-  if (tid == 0) {
-    s_rowpre[0] = 0;
-    s_incpre[0] = 0;
-    for (int r = 0; r < nr; ++r) {
-      s_rowpre[r + 1] += s_rowpre[r];
-      s_incpre[r + 1] += s_incpre[r];
-    }
-  }
-  for (int le = tid; le < ne; le += kAsmThreads) {
-    int dup = 0;
-    for (int a = 0; a < 8; ++a)
-      for (int b = a + 1; b < 8; ++b) dup |= (s_nodes[le * 8 + a] == s_nodes[le * 8 + b]);
-    s_dup[le] = dup;
-  }
-  __syncthreads();
-  const int nslots = s_rowpre[nr];
+  if (WANT_M && tid < 64) s_N[tid] = nshape(tid >> 3, tid & 7);   // s_N[g*8 + k]
 
-  // ---- phase 0b: neighbour and incidence lists into LDS, zero accumulators
-  for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
-    const int32_t i = s_rows[r];
-    const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
-    for (int q = tid % kLanesPerRow; q < co; q += kLanesPerRow) s_re[base + q] = m.re_node[s_sm[r] + q];
-    const int ib = m.ne_ptr[i], b2 = s_incpre[r], ni = s_incpre[r + 1] - b2;
-    for (int q = tid % kLanesPerRow; q < ni; q += kLanesPerRow)
-      s_inc[b2 + q] = (m.ne_local[ib + q] << 3) | (m.ne_list[ib + q] & 7);
-  }
-  if (WANT_K)
-    for (int t = tid; t < nslots * 9; t += kAsmThreads) s_acc[t] = T(0);
-  if (WANT_M)
-    for (int t = tid; t < nslots; t += kAsmThreads) s_accm[t] = 0.0;
-  __syncthreads();
-  // slot of every (incidence, column k) in its row's sorted neighbour list --
-  // the reference's linear search (FEM_model_csr.py:158-160), done once here
-  {
-    const int ninc = s_incpre[nr];
-    for (int t = tid; t < ninc * 8; t += kAsmThreads) {
-      const int inc = t >> 3, k = t & 7;
-      int r = 0, hi = nr - 1;   // row of incidence inc
-      while (r < hi) {
-        const int mid = (r + hi + 1) >> 1;
-        if (s_incpre[mid] <= inc) r = mid; else hi = mid - 1;
-      }
-      const int base = s_rowpre[r], co = s_rowpre[r + 1] - base;
-      const int32_t c = s_nodes[(s_inc[inc] >> 3) * 8 + k];
-      int lo = 0;
-      hi = co - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (s_re[base + mid] < c) lo = mid + 1; else hi = mid;
-      }
-      s_slot[t] = (int16_t)(base + lo);
-    }
-  }
-
-  // ---- phase 1: J^-1 and w at each (element, Gauss point)
-  for (int t = tid; t < ne * 8; t += kAsmThreads) {
-    const int le = t >> 3, g = t & 7;
-    const int e = m.grp_elems[e0 + le];
-    double x[8][3];
+  // ---- phase 1: gradients and weights at each (element, Gauss point)
+  if (p1) {
+    const int32_t nodes[8] = {na.x, na.y, na.z, na.w, nb.x, nb.y, nb.z, nb.w};
+    // 1 +- GS along each axis at this Gauss point: dN_k/dxi_d = s_kd/8 * prod_{a!=d} f_a(k)
+    double fp[3], fm[3];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int32_t nk = s_nodes[le * 8 + k];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) x[k][a] = m.coords[(int64_t)nk * 3 + a];
+    for (int a = 0; a < 3; ++a) {
+      fp[a] = 1.0 + gauss_sign(g, a) * kGS;
+      fm[a] = 1.0 - gauss_sign(g, a) * kGS;
     }
+    const int64_t n0 = nodes[0];
+    const double x00 = m.coords[n0 * 3], x01 = m.coords[n0 * 3 + 1], x02 = m.coords[n0 * 3 + 2];
     double J[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) J[q] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double f[3];
+    for (int k = 1; k < 8; ++k) {   // J = sum_k (x_k - x_0) (x) dN_k (the x_0 term sums to zero)
+      const int64_t nk = nodes[k];
+      const double xr[3] = {m.coords[nk * 3] - x00, m.coords[nk * 3 + 1] - x01, m.coords[nk * 3 + 2] - x02};
+      double dn[3];
+      dn_factored(k, fp, fm, dn);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) f[a] = 1.0 + node_sign(k, a) * gauss_sign(g, a) * kGS;
-      const double dn[3] = {0.125 * node_sign(k, 0) * f[1] * f[2], 0.125 * node_sign(k, 1) * f[0] * f[2],
-                            0.125 * node_sign(k, 2) * f[0] * f[1]};
+      for (int a = 0; a < 3; ++a)
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double xr = x[k][a] - x[0][a];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr, dn[d], J[a * 3 + d]);
-      }
+        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr[a], dn[d], J[a * 3 + d]);
     }
     if (WANT_K) {
       double Ji[9];
       const double det = inv3(J, Ji);
-      const double Ee = young ? young[e] : young_scalar;
-      T* dst = s_ji + (le * 8 + g) * JS;
-#pragma unroll
-      for (int q = 0; q < 9; ++q) dst[q] = (T)Ji[q];
-      dst[9] = (T)(Ee * det);
+      s_w[le * 8 + g] = (T)(Ee * det);
       if (WANT_M) s_wm[le * 8 + g] = rho * det;
+      // recompute the factors rather than keep 24 derivatives live across the inverse
+#pragma unroll
+      for (int a = 0; a < 3; ++a) asm volatile("" : "+v"(fp[a]), "+v"(fm[a]));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        double dn[3];
+        dn_factored(k, fp, fm, dn);
+        T* dst = s_a + (le * 8 + k) * kAStride + 3 * g;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) dst[a] = (T)(dn[0] * Ji[0 * 3 + a] + dn[1] * Ji[1 * 3 + a] + dn[2] * Ji[2 * 3 + a]);
+      }
     } else if (WANT_M) {
       s_wm[le * 8 + g] = rho * det3(J);
     }
   }
   __syncthreads();
 
-  // ---- phase 2: incidence rounds, 16 lanes per row
-  {
-    const int lane = tid % kLanesPerRow;
-    const int k = lane >> 1, h = lane & 1;
-    T dk[4][3];   // DNr_g[k] for this lane's four Gauss points
+  // ---- phase 2: one lane per task, T-lane butterfly per output block
+  const T lh = (T)lhat, mh = (T)mhat;
+  const int lane = tid & 63;
+  for (int base = 0; base < nt; base += kAsmThreads) {   // uniform trip count: all lanes shuffle
+    const int t = base + tid;
+    const bool valid = t < nt;
+    if (base > 0) task = valid ? reinterpret_cast<const int2*>(m.tasks)[t0 + t] : none_task;
+    const int outA = task.x & 0xFFFF, outB = (task.x >> 16) & 0xFFFF;
+    const uint32_t w2 = (uint32_t)task.y;
+    const int c1 = (int)(w2 & 0x3FFF), c2 = (int)((w2 >> 14) & 0x3FFF);
+    const bool dual = (w2 >> 31) & 1u;
+    const int lt = (int)((w2 >> 28) & 7u);
+    const int Tn = dual ? 1 : (1 << lt);
+    const bool has_b = dual && outB != 0xFFFF;
     if (WANT_K) {
+      T G1[9], G2[9];
 #pragma unroll
-      for (int gg = 0; gg < 4; ++gg)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) dk[gg][d] = s_dnr[((4 * h + gg) * 8 + k) * 4 + d];
-    }
-    for (int r = tid / kLanesPerRow; r < nr; r += kRowsPerPass) {
-      const int ib = s_incpre[r], ie = s_incpre[r + 1];
-#pragma unroll 2
-      for (int t = ib; t < ie; ++t) {
-        const int pk = s_inc[t];
-        const int j = pk & 7, le = pk >> 3;
-        const int slot = s_slot[t * 8 + k];
-        T G[9];
-        double ms = 0.0;
-        if (WANT_K) {
-#pragma unroll
-          for (int q = 0; q < 9; ++q) G[q] = T(0);
-#pragma unroll
-          for (int gg = 0; gg < 4; ++gg) {
-            const int g = 4 * h + gg;
-            const T* Ji = s_ji + (le * 8 + g) * JS;
-            const T* dj = s_dnr + (g * 8 + j) * 4;
-            T ji[10];
-#pragma unroll
-            for (int q = 0; q < 10; ++q) ji[q] = Ji[q];
-            const T dj0 = dj[0], dj1 = dj[1], dj2 = dj[2];
-            T u[3], b[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              u[a] = ji[9] * (dj0 * ji[a] + dj1 * ji[3 + a] + dj2 * ji[6 + a]);
-              b[a] = dk[gg][0] * ji[a] + dk[gg][1] * ji[3 + a] + dk[gg][2] * ji[6 + a];
-            }
-#pragma unroll
-            for (int p = 0; p < 3; ++p)
-#pragma unroll
-              for (int q = 0; q < 3; ++q) G[3 * p + q] = fma(u[p], b[q], G[3 * p + q]);
-          }
-#pragma unroll
-          for (int q = 0; q < 9; ++q) G[q] += __shfl_xor(G[q], 1, 64);
-        }
-        if (WANT_M) {
-          const double* wm = s_wm + le * 8;
-#pragma unroll
-          for (int gg = 0; gg < 4; ++gg) {
-            const int g = 4 * h + gg;
-            ms = fma(wm[g] * s_N[g * 8 + j], s_N[g * 8 + k], ms);
-          }
-          ms += __shfl_xor(ms, 1, 64);
-        }
-        if (h == 0) {
-          if (!s_dup[le]) {
-            if (WANT_K) {
-#pragma unroll
-              for (int q = 0; q < 9; ++q) s_acc[slot * 9 + q] += G[q];
-            }
-            if (WANT_M) s_accm[slot] += ms;
-          } else {
-            // a degenerate element repeats a node: add its columns one by one
-            for (int kk = 0; kk < 8; ++kk) {
-              if (kk == k) {
-                if (WANT_K)
-                  for (int q = 0; q < 9; ++q) s_acc[slot * 9 + q] += G[q];
-                if (WANT_M) s_accm[slot] += ms;
-              }
-              __builtin_amdgcn_wave_barrier();
-            }
-          }
-        }
-      }
-    }
-  }
-  __syncthreads();
-
-  // ---- phase 3: write every block of the group's rows once
-  if (WANT_K) {
-    const T lh = (T)lhat, mh = (T)mhat;
-    for (int t = tid; t < nslots * 9; t += kAsmThreads) {
-      const int q9 = t / 9, e9 = t - 9 * q9;
-      const int p = e9 / 3, q = e9 - 3 * p;
-      int lo = 0, hi = nr - 1;   // row of slot q9: last r with s_rowpre[r] <= q9
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_rowpre[mid] <= q9) lo = mid; else hi = mid - 1;
-      }
-      const T* G = s_acc + q9 * 9;
-      T v = lh * G[3 * p + q] + mh * G[3 * q + p];
-      if (p == q) v += mh * (G[0] + G[4] + G[8]);
-      const int64_t sm = s_sm[lo];
-      const int64_t mloc = q9 - s_rowpre[lo];
-      if (LAYOUT == FEM_LAYOUT_BSR) {
-        k_vals[9 * (sm + mloc) + e9] = v;
+      for (int q = 0; q < 9; ++q) G1[q] = G2[q] = T(0);
+      gram<T>(s_a, s_w, c1, G1);
+      gram<T>(s_a, s_w, c2, G2);
+      if (dual) {   // second output of a dual task: no butterfly, store now
+        if (valid && has_b) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outB, G2, lh, mh);
       } else {
-        const int64_t co = s_rowpre[lo + 1] - s_rowpre[lo];
-        k_vals[9 * sm + 3 * p * co + 3 * mloc + q] = v;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) G1[q] += G2[q];
       }
+      for (int sft = 1; sft < 32; sft <<= 1) {
+        if (!__any(sft < Tn)) break;   // wave-uniform: no lane needs a wider butterfly
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          const T other = __shfl_xor(G1[q], sft, 64);
+          if (sft < Tn) G1[q] += other;
+        }
+      }
+      if (valid && (lane & (Tn - 1)) == 0) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outA, G1, lh, mh);
     }
-  }
-  if (WANT_M) {
-    for (int t = tid; t < nslots; t += kAsmThreads) {
-      int lo = 0, hi = nr - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (s_rowpre[mid] <= t) lo = mid; else hi = mid - 1;
+    if (WANT_M) {
+      double m1 = mass_term(s_wm, s_N, c1), m2 = mass_term(s_wm, s_N, c2);
+      if (!dual) m1 += m2;
+      for (int sft = 1; sft < 32; sft <<= 1) {
+        if (!__any(sft < Tn)) break;
+        const double other = __shfl_xor(m1, sft, 64);
+        if (sft < Tn) m1 += other;
       }
-      m_blocks[(int64_t)s_sm[lo] + (t - s_rowpre[lo])] = s_accm[t];
+      if (valid && (lane & (Tn - 1)) == 0) {
+        m_blocks[(int64_t)s_sm[outA >> 8] + (outA & 255)] = m1;
+        if (has_b) m_blocks[(int64_t)s_sm[outB >> 8] + (outB & 255)] = m2;
+      }
     }
   }
 }
@@ -353,8 +280,8 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
 template <typename T, bool K, bool M, int LAYOUT>
 int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, double rho,
                void* k_vals, double* m_blocks, hipStream_t st) {
-  const int eg = m.max_group_elems, rg = m.max_group_rows, sg = m.max_group_slots, ig = m.max_group_incs;
-  const AsmLds L = asm_lds_layout<T>(eg, rg, sg, ig, K, M);
+  const int eg = m.max_group_elems, rg = m.max_group_rows;
+  const AsmLds L = asm_lds_layout<T>(eg, rg, K, M);
   FEM_REQUIRE(L.total <= 160 * 1024, FEM_E_LIMIT,
               "row group needs %d B of LDS (> 160 KB): use smaller row groups", L.total);
   auto kern = hex8_assemble_kernel<T, K, M, LAYOUT>;
@@ -363,7 +290,7 @@ int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, dou
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
   hipLaunchKernelGGL(kern, dim3((unsigned)m.num_groups), dim3(kAsmThreads), L.total, st, m, young,
-                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg, sg, ig);
+                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -448,12 +375,15 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
   FEM_REQUIRE(m->sm_node && m->co_node && (m->nnzb == 0 || m->re_node), FEM_E_INVALID,
               "pattern pointers are NULL");
   if (need_groups) {
-    FEM_REQUIRE(m->coords && m->elem && m->ne_ptr && m->ne_list && m->grp_row_ptr && m->grp_rows &&
-                    m->grp_elem_ptr && m->grp_elems && m->ne_local,
-                FEM_E_INVALID, "mesh lacks coordinates, incidences or row groups");
-    FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_rows >= 0 &&
-                    m->max_group_slots >= 0 && m->max_group_incs >= 0,
-                FEM_E_INVALID, "bad group sizes");
+    FEM_REQUIRE(m->coords && m->elem && m->grp_row_ptr && m->grp_rows && m->grp_elem_ptr &&
+                    m->grp_elems && m->grp_enodes && m->grp_task_ptr &&
+                    (m->max_group_tasks == 0 || m->tasks),
+                FEM_E_INVALID, "mesh lacks coordinates, row groups or assembly tasks");
+    FEM_REQUIRE(m->num_groups >= 0 && m->max_group_elems >= 0 && m->max_group_elems * 8 <= kAsmThreads &&
+                    m->max_group_rows >= 0 && m->max_group_rows <= kAsmThreads && m->max_group_rows < 255,
+                FEM_E_LIMIT, "row groups must have at most %d elements and 254 rows", kAsmThreads / 8);
+    FEM_REQUIRE(((uintptr_t)m->grp_enodes & 15) == 0 && ((uintptr_t)m->tasks & 7) == 0, FEM_E_INVALID,
+                "grp_enodes must be 16-byte and tasks 8-byte aligned");
   }
   return FEM_OK;
 }
@@ -464,13 +394,9 @@ int check_mesh(const fem_mesh* m, bool need_groups) {
 using namespace fem;
 
 extern "C" int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
-                                               int32_t max_group_slots, int32_t max_group_incs,
                                                int k_dtype, int want_k, int want_m) {
-  if (k_dtype == FEM_DTYPE_F32)
-    return asm_lds_layout<float>(max_group_elems, max_group_rows, max_group_slots, max_group_incs, want_k,
-                                 want_m).total;
-  return asm_lds_layout<double>(max_group_elems, max_group_rows, max_group_slots, max_group_incs, want_k,
-                                want_m).total;
+  if (k_dtype == FEM_DTYPE_F32) return asm_lds_layout<float>(max_group_elems, max_group_rows, want_k, want_m).total;
+  return asm_lds_layout<double>(max_group_elems, max_group_rows, want_k, want_m).total;
 }
 
 extern "C" int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_scalar,

@@ -92,3 +92,124 @@ def build_row_groups(N, E, ne_ptr, ne_list, gid, max_elems, co_node=None):
                 max_group_rows=int(np.diff(grp_row_ptr).max()) if G else 0,
                 max_group_slots=int(slots.max()) if G else 0,
                 max_group_incs=int(incs.max()) if G else 0)
+
+
+def _next_pow2(a):
+    a = np.maximum(a, 1)
+    return (1 << np.ceil(np.log2(a)).astype(np.int64)).astype(np.int64)
+
+
+def build_tasks(N, elem, sm_node, re_node, ne_ptr, ne_list, groups):
+    """Contribution tasks of the row-owner assembly kernel (one-time symbolic work).
+
+    Every output block (row i, slot m) is the sum over the elements e that
+    contain both node i (local index j) and the column node (local index k)
+    of the element Gram block G_e(j, k).  The contributions of a block are
+    split into T = next_pow2(ceil(n/2)) tasks of at most two contributions;
+    the T lanes of one block are adjacent and T-aligned (tasks of a group are
+    ordered by decreasing T), so the kernel combines them with a fixed
+    butterfly of lane shuffles.  Blocks with a single contribution are paired
+    into "dual" tasks with two outputs.
+
+    Returns grp_task_ptr (G+1) and tasks (ntask, 2) int32:
+      [0] out_a | out_b << 16, out = row_local << 8 | m (m: slot within the
+          row); out_b = 0xFFFF unless the task is dual
+      [1] c1 | c2 << 14 | log2(T) << 28 | dual << 31, c = le << 6 | j << 3 | k,
+          c = 0x3FFF when absent.
+    """
+    ne_ptr = np.asarray(ne_ptr, dtype=np.int64)
+    elem = np.asarray(elem, dtype=np.int64)
+    sm_node = np.asarray(sm_node, dtype=np.int64)
+    G = int(groups["num_groups"])
+    grp_row_ptr = groups["grp_row_ptr"].astype(np.int64)
+    grp_rows = groups["grp_rows"].astype(np.int64)
+    gid = np.repeat(np.arange(G, dtype=np.int64), np.diff(grp_row_ptr))
+    g_of_node = np.empty(N, dtype=np.int64)
+    r_of_node = np.empty(N, dtype=np.int64)
+    g_of_node[grp_rows] = gid
+    r_of_node[grp_rows] = np.arange(N, dtype=np.int64) - grp_row_ptr[gid]
+    if int(np.diff(grp_row_ptr).max(initial=0)) > 255:
+        raise ValueError("row groups of more than 255 rows")
+    inc_node = np.repeat(np.arange(N, dtype=np.int64), np.diff(ne_ptr))
+    ne_list = np.asarray(ne_list, dtype=np.int64)
+    inc_e = ne_list >> 3
+    inc_j = ne_list & 7
+    inc_le = np.asarray(groups["ne_local"], dtype=np.int64)
+    if inc_le.size and inc_le.max() > 254:
+        raise ValueError("more than 255 elements in a row group")
+    # expand to (incidence, k)
+    I = len(inc_node)
+    node = np.repeat(inc_node, 8)
+    k = np.tile(np.arange(8, dtype=np.int64), I)
+    e = np.repeat(inc_e, 8)
+    col = elem[e, k]
+    rows = np.repeat(np.arange(N, dtype=np.int64), np.diff(sm_node))
+    keys = rows * N + np.asarray(re_node, dtype=np.int64)
+    slot = np.searchsorted(keys, node * N + col)
+    m = slot - sm_node[node]
+    if m.size and m.max() > 255:
+        raise ValueError("a row with more than 256 neighbour blocks")
+    g = g_of_node[node]
+    r = r_of_node[node]
+    cc = (np.repeat(inc_le, 8) << 6) | (np.repeat(inc_j, 8) << 3) | k
+    order = np.lexsort((e, m, r, g))
+    g, r, m, cc = g[order], r[order], m[order], cc[order]
+    out_key = (r << 8) | m
+    # runs of equal (g, out_key) = one output block
+    skey = g * 65536 + out_key
+    start = np.nonzero(np.concatenate([[True], skey[1:] != skey[:-1]]))[0]
+    n = np.diff(np.concatenate([start, [len(skey)]]))
+    if n.size and n.max() > 64:
+        raise ValueError("a node pair shared by more than 64 elements")
+    T = _next_pow2((n + 1) // 2)
+    sg, sout = g[start], out_key[start]
+    NONE = 0x3FFF
+    recs = []   # arrays of (g, sortkey, outA, outB, word2)
+    multi = n >= 2
+    # non-single blocks: T tasks each, task q holds contributions 2q, 2q+1
+    ms = np.nonzero(multi)[0]
+    if len(ms):
+        Tm = T[ms]
+        tb = np.repeat(ms, Tm)
+        q = np.arange(Tm.sum()) - np.repeat(np.cumsum(Tm) - Tm, Tm)
+        i1 = start[tb] + 2 * q
+        i2 = i1 + 1
+        has1 = 2 * q < n[tb]
+        has2 = 2 * q + 1 < n[tb]
+        c1 = np.where(has1, cc[np.minimum(i1, len(cc) - 1)], NONE)
+        c2 = np.where(has2, cc[np.minimum(i2, len(cc) - 1)], NONE)
+        lt = np.log2(T[tb]).astype(np.int64)
+        w2 = c1 | (c2 << 14) | (lt << 28)
+        # decreasing T inside a group keeps every block's lanes T-aligned
+        sortk = (20 - lt) * (1 << 24) + tb
+        recs.append((sg[tb], sortk * 64 + q, sout[tb], np.full(len(tb), -1), w2))
+    ss = np.nonzero(~multi)[0]
+    if len(ss):
+        # pair singles of the same group into dual tasks
+        gs = sg[ss]
+        o = np.argsort(gs, kind="stable")
+        ss, gs = ss[o], gs[o]
+        first = np.concatenate([[True], gs[1:] != gs[:-1]])
+        pos = np.arange(len(ss)) - np.maximum.accumulate(np.where(first, np.arange(len(ss)), 0))
+        a_idx = np.nonzero(pos % 2 == 0)[0]
+        b_idx = a_idx + 1
+        b_ok = (b_idx < len(ss))
+        b_ok[b_ok] &= (gs[b_idx[b_ok]] == gs[a_idx[b_ok]])
+        cA = cc[start[ss[a_idx]]]
+        cB = np.where(b_ok, cc[start[ss[np.minimum(b_idx, len(ss) - 1)]]], NONE)
+        oB = np.where(b_ok, sout[ss[np.minimum(b_idx, len(ss) - 1)]], -1)
+        w2 = cA | (cB << 14) | (1 << 31)
+        # after every multi-task block (T = 1 needs no alignment)
+        sortk = np.full(len(a_idx), 21 << 30, dtype=np.int64) + a_idx
+        recs.append((gs[a_idx], sortk, sout[ss[a_idx]], oB, w2))
+    gg = np.concatenate([x[0] for x in recs])
+    sk = np.concatenate([x[1] for x in recs])
+    oa = np.concatenate([x[2] for x in recs])
+    ob = np.concatenate([x[3] for x in recs])
+    w2 = np.concatenate([x[4] for x in recs])
+    o = np.lexsort((sk, gg))
+    w0 = oa[o] | (np.where(ob[o] < 0, 0xFFFF, ob[o]) << 16)
+    tasks = np.stack([w0, w2[o]], axis=1).astype(np.int64)
+    tasks = tasks.astype(np.uint32).view(np.int32)
+    grp_task_ptr = np.concatenate([[0], np.cumsum(np.bincount(gg, minlength=G))]).astype(np.int32)
+    return grp_task_ptr, np.ascontiguousarray(tasks)

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .groups import DEFAULT_BRICK, build_row_groups, node_group_ids
+from .groups import DEFAULT_BRICK, build_row_groups, build_tasks, node_group_ids
 from .runtime import device, stream_ptr
 
 # Upper bound on the elements one row group may touch (LDS staging budget).
@@ -34,24 +34,25 @@ class DeviceMesh:
         self.t["elem"] = torch.as_tensor(elem, dtype=torch.int32).contiguous().to(dev)
         for k, v in pattern.items():
             self.t[k] = v
-        for k in ("grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "ne_local"):
-            self.t[k] = torch.as_tensor(groups[k], dtype=torch.int32).to(dev)
+        for k in ("grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "grp_enodes", "grp_task_ptr", "tasks"):
+            self.t[k] = torch.as_tensor(groups[k], dtype=torch.int32).contiguous().to(dev)
         self.num_node = int(coords.shape[0])
         self.num_elem = int(elem.shape[0])
         self.nnzb = int(pattern["re_node"].numel())
         m = _lib.FemMesh()
         m.num_node, m.num_elem, m.nnzb = self.num_node, self.num_elem, self.nnzb
         for name in ("coords", "elem", "sm_node", "re_node", "co_node", "co_elem", "ne_ptr", "ne_list",
-                     "grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "ne_local"):
+                     "grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "grp_enodes", "grp_task_ptr",
+                     "tasks"):
             setattr(m, name, self.t[name].data_ptr() if self.t[name].numel() else None)
         m.num_groups = int(groups["num_groups"])
         m.max_group_elems = int(groups["max_group_elems"])
         m.max_group_rows = int(groups["max_group_rows"])
         m.max_group_slots = int(groups["max_group_slots"])
-        m.max_group_incs = int(groups["max_group_incs"])
+        m.max_group_tasks = int(groups["max_group_tasks"])
         self.c = m
         self.groups_info = {k: groups[k] for k in ("num_groups", "max_group_elems", "max_group_rows",
-                                                   "max_group_slots", "max_group_incs")}
+                                                   "max_group_slots", "max_group_tasks")}
 
     @property
     def ref(self):
@@ -134,8 +135,11 @@ class Physics:
         gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=self.brick)
         groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, MAX_GROUP_ELEMS,
                                   co_node=self.CoNode)
+        tp, tasks = build_tasks(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr, ne_list, groups)
+        groups["grp_task_ptr"], groups["tasks"] = tp, tasks
+        groups["grp_enodes"] = np.ascontiguousarray(self.elem_nodes[groups["grp_elems"]].reshape(-1))
+        groups["max_group_tasks"] = int(np.diff(tp).max()) if len(tp) > 1 else 0
         lds = _lib.lib().fem_hex8_assemble_lds_bytes(groups["max_group_elems"], groups["max_group_rows"],
-                                                      groups["max_group_slots"], groups["max_group_incs"],
                                                       _lib.DTYPE_F64, 1, 1)
         if lds > 160 * 1024:
             raise ValueError(f"row groups need {lds} B of LDS; use a smaller brick")

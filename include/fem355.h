@@ -56,8 +56,16 @@ int fem_device_count(int* count);
  * Mesh/pattern descriptor: all pointers are device pointers.
  * Row groups (grp_*) partition the node rows into spatially compact sets
  * whose incident elements are staged together in LDS by the row-owner
- * kernels; ne_local[t] is the index, within the group of ne_list[t]'s node,
- * of element ne_list[t]/8 in that group's sorted element list.
+ * kernels; grp_enodes holds the 8 node indices of every element of every
+ * group (elem[grp_elems[t]], so the kernel reaches coordinates in one
+ * indirection).  tasks are the one-time symbolic contribution lists of the
+ * assembly kernel: per group, each task names up to two element
+ * contributions (local element, local row node j, local column node k) of
+ * one output block (row-local index << 8 | slot in the row), the number T of
+ * adjacent lanes sharing that block, or a second output block for "dual"
+ * tasks.  Task word 0: out_a | out_b << 16 (out_b = 0xFFFF: none); word 1:
+ * c1 | c2 << 14 | log2(T) << 28 | dual << 31 with c = elem << 6 | j << 3 | k
+ * (0x3FFF: none).
  * ---------------------------------------------------------------------- */
 typedef struct fem_mesh {
   int64_t num_node;
@@ -75,12 +83,14 @@ typedef struct fem_mesh {
   int32_t max_group_elems;     /* max over groups of the element count */
   int32_t max_group_rows;      /* max over groups of the row count     */
   int32_t max_group_slots;     /* max over groups of sum CoNode of its rows */
-  int32_t max_group_incs;      /* max over groups of the rows' incidence count */
+  int32_t max_group_tasks;     /* max over groups of the task count */
   const int32_t* grp_row_ptr;  /* G + 1 */
   const int32_t* grp_rows;     /* N, rows in group order */
   const int32_t* grp_elem_ptr; /* G + 1 */
   const int32_t* grp_elems;    /* sorted element ids per group */
-  const int32_t* ne_local;     /* ne_ptr[N] */
+  const int32_t* grp_enodes;   /* 8 per entry of grp_elems, 16-byte aligned */
+  const int32_t* grp_task_ptr; /* G + 1 */
+  const int32_t* tasks;        /* 2 int32 per task, 8-byte aligned */
 } fem_mesh;
 
 /* ---- a1: symbolic pattern (replaces READ_physics.py:42-61 pre_process) --- */
@@ -117,8 +127,8 @@ int fem_csr_values_M(const fem_mesh* mesh, const double* m_blocks, double* val_m
 /* ---- a4-a7: hex8 stiffness / mass assembly (replaces InstallMatrixK
  *      FEM_model_csr.py:150-165, InstallMatrixKL FEM_model_bsr.py:163-175,
  *      InstallMatrixM FEM_model_csr.py:178-192 with UnitMatrixK/M) ---------
- * Row-owner, atomic-free: each node row's blocks are summed in a fixed
- * element order and written exactly once.  young: per-element modulus (E)
+ * Row-owner, atomic-free: each output block is summed in a fixed order
+ * (a lane-shuffle butterfly over its tasks) and written exactly once.  young: per-element modulus (E)
  * or NULL to use young_scalar.  k_vals (nullable): 9*nnzb values of dtype
  * k_dtype in k_layout.  m_blocks (nullable): nnzb f64 scalars, the mass
  * coupling of each node pair (the value the reference replicates on the
@@ -129,8 +139,7 @@ int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_sc
 
 /* LDS bytes one work-group of fem_hex8_assemble uses for a row group of the
  * given sizes (to check a grouping fits the 160 KB of a CU before launch). */
-int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows,
-                                    int32_t max_group_slots, int32_t max_group_incs, int k_dtype,
+int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows, int k_dtype,
                                     int want_k, int want_m);
 
 /* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and

@@ -1,0 +1,162 @@
+"""Host logic of the row-owner assembly: row groups and contribution tasks.
+
+The assembly kernel's phase 2 is emulated here lane by lane (64-lane waves,
+the T-lane shuffle butterfly with its wave-uniform early exit, head-lane
+stores), with the oracle's element blocks standing in for the Gram blocks the
+kernel builds in LDS.  Every output block must be written exactly once and
+equal the oracle's assembled K (and M).  CPU only.
+"""
+import numpy as np
+import pytest
+
+from oracle import fem_oracle as O
+from fem_taichi_amd import groups as Gr
+from fem_taichi_amd import mesh
+
+NONE = 0x3FFF
+
+
+def incidences(en, N):
+    """ne_ptr / ne_list (e << 3 | j, sorted per node) of an (E, 8) element table."""
+    e = np.repeat(np.arange(len(en)), 8)
+    j = np.tile(np.arange(8), len(en))
+    node = en.reshape(-1)
+    o = np.lexsort((e * 8 + j, node))
+    ne_ptr = np.concatenate([[0], np.cumsum(np.bincount(node, minlength=N))])
+    return ne_ptr, (e * 8 + j)[o]
+
+
+def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512):
+    """Phase 2 of the kernel on element blocks ``blocks_e`` (E, 8, 8, ...)."""
+    out = np.zeros((nslots,) + blocks_e.shape[3:])
+    hits = np.zeros(nslots, dtype=np.int64)
+    gep, ge = groups["grp_elem_ptr"], groups["grp_elems"]
+    grp, grows = groups["grp_row_ptr"], groups["grp_rows"]
+    tw = tasks.view(np.uint32)
+    zero = np.zeros(blocks_e.shape[3:])
+
+    def contrib(g, c):
+        if c == NONE:
+            return zero
+        le, j, k = c >> 6, (c >> 3) & 7, c & 7
+        assert le < gep[g + 1] - gep[g]
+        return blocks_e[ge[gep[g] + le], j, k]
+
+    for g in range(len(tp) - 1):
+        t0, nt = tp[g], tp[g + 1] - tp[g]
+        for base in range(0, nt, threads):
+            for w0 in range(0, threads, wave):
+                lanes = range(w0, w0 + wave)
+                val, T, outA, outB, dual, c2v, valid = [], [], [], [], [], [], []
+                for tid in lanes:
+                    t = base + tid
+                    ok = t < nt
+                    rec = tw[t0 + t] if ok else np.array([0xFFFF0000, NONE | NONE << 14], dtype=np.uint32)
+                    w2 = int(rec[1])
+                    c1, c2 = w2 & 0x3FFF, (w2 >> 14) & 0x3FFF
+                    d = bool(w2 >> 31)
+                    Tn = 1 if d else 1 << ((w2 >> 28) & 7)
+                    v1, v2 = contrib(g, c1), contrib(g, c2)
+                    val.append(v1 if d else v1 + v2)
+                    c2v.append(v2)
+                    T.append(Tn)
+                    outA.append(int(rec[0]) & 0xFFFF)
+                    ob = int(rec[0]) >> 16
+                    outB.append(-1 if ob == 0xFFFF else ob)
+                    dual.append(d)
+                    valid.append(ok)
+                s = 1
+                while s < 32:
+                    if not any(s < x for x in T):
+                        break
+                    new = list(val)
+                    for li in range(wave):
+                        if s < T[li]:
+                            new[li] = val[li] + val[li ^ s]
+                    val = new
+                    s <<= 1
+                for li in range(wave):
+                    if not valid[li] or (li & (T[li] - 1)) != 0:
+                        continue
+                    for o, v in [(outA[li], val[li])] + ([(outB[li], c2v[li])] if dual[li] and outB[li] >= 0 else []):
+                        r, m = o >> 8, o & 255
+                        node = grows[grp[g] + r]
+                        slot = SmNode[node] + m
+                        out[slot] = v
+                        hits[slot] += 1
+    return out, hits
+
+
+def check_block_lanes(tasks, tp):
+    """The T tasks of a block are adjacent, T-aligned within their group, share outA."""
+    w2 = tasks[:, 1].view(np.uint32)
+    outa = tasks[:, 0].view(np.uint32) & 0xFFFF
+    for g in range(len(tp) - 1):
+        t, end = tp[g], tp[g + 1]
+        while t < end:
+            d = bool(w2[t] >> 31)
+            Tn = 1 if d else 1 << ((int(w2[t]) >> 28) & 7)
+            assert (t - tp[g]) % Tn == 0
+            assert np.all(outa[t:t + Tn] == outa[t])
+            t += Tn
+
+
+CASES = [
+    ("brick", dict(dims=(9, 5, 6), jitter=0.1, seed=3), True),
+    ("morton", dict(dims=(7, 6, 5), jitter=0.15, seed=4), False),
+    ("permuted", dict(dims=(5, 4, 7), jitter=0.1, seed=5), False),
+]
+
+
+@pytest.mark.parametrize("name,kw,brick", CASES, ids=[c[0] for c in CASES])
+def test_tasks_assemble_like_oracle(name, kw, brick):
+    g = mesh.structured_block(*kw["dims"], 1.0, kw["jitter"], kw["seed"], E_sigma=0.3)
+    Node, Elem = (mesh.permute_ids(g, seed=kw["seed"]) if name == "permuted" else (g.Node, g.Elem))
+    pp = O.pre_process(Node, Elem)
+    en = O.element_nodes(Elem, pp["Nind"])
+    N = Node.shape[1]
+    coords = np.asarray(Node[-3:], dtype=np.float64).T
+    ne_ptr, ne_list = incidences(en, N)
+    if brick:
+        nx, ny, nz = kw["dims"]
+        gid = Gr.node_group_ids(N, node_dims=(nx + 1, ny + 1, nz + 1))
+    else:
+        gid = Gr.node_group_ids(N, xyz=coords)
+    groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 48, co_node=pp["CoNode"])
+    assert groups["max_group_elems"] <= 48
+    tp, tasks = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    check_block_lanes(tasks, tp)
+
+    Coord = O.element_coords(Node, Elem, pp["Nind"])
+    EX = g.Materials["EX"]
+    Ke = O.element_K(Coord, EX, g.Materials["PO"])
+    nnzb = int(pp["SmNode"][-1])
+    Kt, hits = emulate(tasks, tp, groups, Ke, pp["SmNode"], nnzb)
+    assert np.all(hits == 1)
+    Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
+    err = np.linalg.norm(Kt.reshape(-1) - Kref.reshape(-1)) / np.linalg.norm(Kref)
+    assert err < 1e-14, err
+
+    Me = O.element_M(Coord, g.Materials["DE"])
+    Mt, hits = emulate(tasks, tp, groups, Me, pp["SmNode"], nnzb)
+    assert np.all(hits == 1)
+    Mref = O.assemble_M_block(Me, en, pp["SmNode"], pp["ReNode"])
+    assert np.linalg.norm(Mt - Mref) / np.linalg.norm(Mref) < 1e-14
+
+
+def test_tasks_reject_crowded_node_pair():
+    # 65 elements on the same 8 nodes: a node pair with 65 contributions cannot
+    # be summed by a 32-lane butterfly of pairs
+    N = 8
+    en = np.tile(np.arange(8, dtype=np.int64), (65, 1))
+    rows = [set() for _ in range(N)]
+    for nodes in en:
+        for a in nodes:
+            rows[a].update(int(b) for b in nodes)
+    sm = np.concatenate([[0], np.cumsum([len(r) for r in rows])])
+    re = np.concatenate([sorted(r) for r in rows])
+    ne_ptr, ne_list = incidences(en, N)
+    gid = np.zeros(N, dtype=np.int64)
+    groups = Gr.build_row_groups(N, 65, ne_ptr, ne_list, gid, 200)
+    with pytest.raises(ValueError, match="more than 64"):
+        Gr.build_tasks(N, en, sm, re, ne_ptr, ne_list, groups)
