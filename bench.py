@@ -42,7 +42,22 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time the static solver forms")
     ap.add_argument("--solve-mesh", default="400x50x50")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a plain 256 MiB device copy (the known-size kernel that calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_probe(nbytes=256 << 20, reps=5):
+    """A plain streaming kernel over nbytes: dst = 2 * src (reads nbytes,
+    writes nbytes; torch's elementwise kernel)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        torch.mul(src, 2.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"# copy probe: {reps} x (read {nbytes} B, write {nbytes} B)", flush=True)
 
 
 def alg_bytes_K(N, E, nnzb, per_elem_E=True, val_bytes=8):
@@ -93,9 +108,18 @@ def time_solves(mesh_spec):
     out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "tol": 1e-8}
     _, info = solvers.solve_pcg64(K64, F, tol=1e-8)
     out["pcg_fp64_bj"] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual}
+    print(f"# pcg_fp64_bj {json.dumps(out['pcg_fp64_bj'])}", flush=True)
     _, info = solvers.solve_mixed(K64, K32, F, tol=1e-8)
     out["mixed_fp32_pcg_fp64_refine"] = {"seconds": info.seconds, "iterations": info.iterations,
                                          "outer": info.outer, "rel_residual": info.rel_residual}
+    print(f"# mixed {json.dumps(out['mixed_fp32_pcg_fp64_refine'])}", flush=True)
+    from fem_taichi_amd.learned import solve_learned
+    for key, use_net, matrix in (("learned_fcg_f32_matrix", True, "f32"), ("learned_fcg_f64_matrix", True, "f64"),
+                                 ("fcg_f64_matrix_bj_only", False, "f64")):
+        _, info = solve_learned(model, K64, F, tol=1e-8, use_net=use_net, matrix=matrix)
+        out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual,
+                    "fell_back": bool(info.history[0]["fell_back"])}
+        print(f"# {key} {json.dumps(out[key])}", flush=True)
     print(f"# solves {json.dumps(out)}", flush=True)
     return out
 
@@ -175,6 +199,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(g, phys)
     if args.solve:
         result["static_solve"] = time_solves(args.solve_mesh)
+    if args.copy_probe:
+        copy_probe()
     print(json.dumps(result), flush=True)
 
 

@@ -8,10 +8,10 @@ backward passes run in the hand-written BF16 MFMA kernels of meshconv.hip
 (FP32 accumulation); master weights stay FP32 and are cast to BF16 for
 every pass.  The preconditioner is z = Dinv r + s(r) * net(r / s(r)) / E_ref
 (block-Jacobi plus the learned correction, s = RMS of r), applied inside
-truncated flexible CG (FCG(m), FP32 SpMV on the FP32 copy of K with FP64
-vectors and FP64 residual replacement, as the mixed form).  If the
-FP64 residual stops improving for ``patience`` checks, or turns non-finite,
-the solve falls back to block-Jacobi PCG from the current iterate.
+truncated flexible CG (FCG(m); SpMV on the FP32 copy of K with FP64 vectors,
+or on the FP64 K; FP64 residual replacement).  If the FP64 residual stops
+improving over ``patience`` replacements, or turns non-finite, the solve
+falls back to FP64 block-Jacobi PCG from its best iterate.
 
 Training data are the residual/error pairs of plain block-Jacobi PCG on
 seeded slabs: the residuals r_k at selected iterations k of a PCG run and
@@ -236,22 +236,32 @@ class LearnedPreconditioner:
         return c
 
 
-def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, drop=1e-2, patience=25,
-              mmax=4, trace=None):
+def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, drop=1e-2, patience=4,
+              stagnation=25, mmax=4, trace=None):
     """Truncated flexible CG (FCG(m), Notay 2000): each new direction is
     M-orthogonalised (in the K inner product) against the last ``mmax``
     directions, which keeps CG-like convergence with a nonlinear
-    preconditioner.  FP32 SpMV of K32 on FP64 vectors, FP64 residual
-    replacement (reliable updates), block-Jacobi PCG fallback when the FP64
-    residual stops improving for ``patience`` checks or turns non-finite.
+    preconditioner.  The SpMV streams ``K32`` (FP32 values, FP64 vectors) when
+    it is given, else the FP64 ``K64``.  The recursive residual is replaced by
+    the FP64 ``F - K64 x`` whenever it has dropped by ``drop`` or claims
+    convergence (reliable updates).  The flexible iteration is abandoned for
+    FP64 block-Jacobi PCG (from its best iterate, or from zero; with its own
+    ``maxiter``) when ``patience`` consecutive replacements fail to halve the
+    best FP64 residual, when the recursive residual sets no new low (by 10%)
+    in ``stagnation`` consecutive checks, or when it turns non-finite.
     Returns (x, info dict)."""
     from . import solvers
-    mesh_ref = K32.mesh.ref
     st = stream_ptr()
+    if K32 is not None:
+        mesh_ref = K32.mesh.ref
 
-    def spmv32(x, y):
-        _lib.call("fem_bsr_spmv", mesh_ref, K32.vals.data_ptr(), _lib.DTYPE_F32F64, x.data_ptr(), y.data_ptr(), st)
-        return y
+        def spmv(x, y):
+            _lib.call("fem_bsr_spmv", mesh_ref, K32.vals.data_ptr(), _lib.DTYPE_F32F64, x.data_ptr(),
+                      y.data_ptr(), st)
+            return y
+    else:
+        def spmv(x, y):
+            return K64.matvec(x, y)
 
     x = torch.zeros_like(F)
     r = F.clone()
@@ -259,14 +269,15 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
     hist = []          # (p_j, q_j = K p_j, p_j . q_j)
     it = 0
     rr_anchor = bb
-    best = bb
-    stall = 0
-    while it < maxiter:
+    best_rr, best_x, stalls = bb, None, 0
+    low_rr, flat = bb, 0
+    rel = 1.0 if bb > 0 else 0.0
+    while it < maxiter and rel >= tol:
         z = precond(r)
         p = z
         for pj, qj, pqj in hist:
             p = p - (torch.dot(z, qj) / pqj) * pj
-        q = spmv32(p.contiguous(), torch.empty_like(F))
+        q = spmv(p.contiguous(), torch.empty_like(F))
         pq = torch.dot(p, q)
         alpha = torch.dot(p, r) / pq
         x.add_(alpha * p)
@@ -275,34 +286,41 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
         if len(hist) > mmax:
             hist.pop(0)
         it += 1
-        if it % check_every == 0:
+        if it % check_every:
+            continue
+        rr = float(torch.dot(r, r))
+        if not math.isfinite(rr):
+            break
+        if rr < 0.81 * low_rr:
+            low_rr, flat = rr, 0
+        else:
+            flat += 1
+            if flat >= stagnation:
+                break
+        rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
+        if rel < tol or rr < drop * drop * rr_anchor:
+            r = F - K64.matvec(x)
             rr = float(torch.dot(r, r))
+            rr_anchor = rr
+            rel = (rr / bb) ** 0.5
+            if trace is not None:
+                trace.append((it, rel))
             if not math.isfinite(rr):
                 break
-            rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
-            if rel < tol or rr < drop * drop * rr_anchor:
-                r = F - K64.matvec(x)
-                rr = float(torch.dot(r, r))
-                rr_anchor = rr
-                rel = (rr / bb) ** 0.5
-                if trace is not None:
-                    trace.append((it, rel))
-                if rel < tol:
-                    break
-            if rr < 0.9 * best:
-                best, stall = rr, 0
+            if rr < 0.25 * best_rr:
+                best_rr, best_x, stalls = rr, x.clone(), 0
             else:
-                stall += 1
-            if stall >= patience:
-                break
+                stalls += 1
+                if stalls >= patience:
+                    break
     fell_back = False
     rel = float(torch.linalg.vector_norm(F - K64.matvec(x)) / math.sqrt(bb)) if bb > 0 else 0.0
-    if not (math.isfinite(rel) and rel < 1.0):
-        x = torch.zeros_like(F)      # the flexible iterate is no better than zero: restart
     if not (rel < tol):
         fell_back = True
-        dinv32, _ = K32.block_jacobi()
-        x, it2, _ = solvers.pcg(K32, F, dinv32, tol, maxiter, x0=x.clone(), exact=K64)
+        if not (math.isfinite(rel) and rel < 1.0):
+            x = best_x if best_x is not None else torch.zeros_like(F)
+        dinv64, _ = K64.block_jacobi()
+        x, it2, _ = solvers.pcg(K64, F, dinv64, tol, maxiter, x0=x.clone())
         x = x.clone()
         it += it2
         rel = float(torch.linalg.vector_norm(F - K64.matvec(x)) / math.sqrt(bb))
@@ -310,8 +328,11 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
                "applications": getattr(precond, "applications", None)}
 
 
-def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=100000, use_net=True, **kw):
-    """Static solve with the learned preconditioner (structured meshes)."""
+def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=100000, use_net=True,
+                  matrix="f32", **kw):
+    """Static solve with the learned preconditioner (structured meshes).
+    ``matrix``: "f32" streams the FP32 copy of K in the flexible iteration,
+    "f64" the FP64 one."""
     from . import solvers
     phys = model.Physics
     if getattr(phys, "node_dims", None) is None:
@@ -322,7 +343,7 @@ def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=10000
         if weights is None:
             weights = default_weights_path()
         net.load_state_dict(torch.load(weights, weights_only=True))
-    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32"))
+    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32")) if matrix == "f32" else None
     dinv64, _ = K64.block_jacobi()
     pre = LearnedPreconditioner(net, phys.node_dims, node_log_modulus(phys, model.EX), dinv64,
                                 free_mask=free_dof_mask(model), use_net=use_net)

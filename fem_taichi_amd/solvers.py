@@ -17,6 +17,7 @@ residual ||F - K U|| / ||F|| on the penalised K, as the reference prints it
 from __future__ import annotations
 
 import dataclasses
+import math
 import time
 
 import torch
@@ -79,7 +80,8 @@ class PcgWorkspace:
 
 def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxiter: int,
         ws: PcgWorkspace | None = None, check_every: int = 16, x0: torch.Tensor | None = None,
-        exact: BsrOperator | None = None, drop: float = 1e-2, trace: list | None = None):
+        exact: BsrOperator | None = None, drop: float = 1e-2, trace: list | None = None,
+        patience: int = 4, status: dict | None = None):
     """Block-Jacobi PCG to ||r||/||b|| < tol.
 
     ``op`` supplies the matrix used inside the iteration.  When ``exact`` (an
@@ -89,8 +91,16 @@ def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxite
     replacement or claims convergence, it is replaced by the exact
     ``b - exact @ x`` (reliable updates, van der Vorst & Ye 2000), which keeps
     the gap between recursion and true residual small.  If the exact residual
-    still exceeds the recursive one by more than 2x the search direction
+    still exceeds the recursive one by more than 4x the search direction
     restarts (p = z).  The stopping test is thus on the FP64 residual.
+
+    The FP32 matrix differs from the FP64 one by ~6e-8 per entry; when the
+    conditioning of K makes that perturbation comparable to its smallest
+    eigenvalues the FP64 residual stops falling however long the recursion
+    runs.  In mixed mode the iteration therefore stops once ``patience``
+    consecutive replacements fail to reduce the best FP64 residual norm by 2x
+    (or it turns non-finite); ``status`` (a dict, if given) then receives
+    "stalled": True and "best_x", the iterate with the smallest FP64 residual.
     Returns (x, iterations, relative residual).
     """
     N = op.mesh.num_node
@@ -127,12 +137,27 @@ def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxite
     it = 0
     rr, bb = read()
     rr_anchor = rr
+    best_rr, best_x, stalls = rr, None, 0
+    if status is not None:
+        status["stalled"] = False
     while True:
         rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
         if mixed and it > 0 and (rel < tol or rr < drop * drop * rr_anchor):
             rr, bb = replace(rr)
             rr_anchor = rr
             rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
+            if math.isfinite(rr) and rr < 0.25 * best_rr:
+                best_rr, stalls = rr, 0
+                if status is not None:
+                    best_x = ws.x.clone()
+            else:
+                stalls += 1
+            if not math.isfinite(rr) or (stalls >= patience and rel >= tol):
+                if status is not None:
+                    status["stalled"] = True
+                    status["best_x"] = best_x
+                    status["best_rel"] = (best_rr / bb) ** 0.5 if bb > 0 else 0.0
+                break
         if rel < tol or bb == 0 or it >= maxiter:
             break
         for _ in range(min(check_every, maxiter - it)):
@@ -162,14 +187,31 @@ def solve_pcg64(K64: BsrOperator, F: torch.Tensor, tol=1e-8, maxiter=200000):
 
 
 def solve_mixed(K64: BsrOperator, K32: BsrOperator, F: torch.Tensor, tol=1e-8, drop=1e-2,
-                maxiter=200000):
+                maxiter=200000, patience=4):
     """PCG whose SpMV streams the FP32 copy of K while vectors, dot products
-    and the block-Jacobi application stay FP64; every ``replace_every``
-    iterations (and before stopping) the residual is recomputed in FP64 on the
-    FP64 K, so the stopping test is the FP64 relative residual."""
+    and the block-Jacobi application stay FP64; the residual is replaced by
+    the FP64 one on the FP64 K (reliable updates), so the stopping test is the
+    FP64 relative residual.  If the FP64 residual stalls (the FP32 rounding
+    of K is too large for K's conditioning) the solve finishes with FP64 PCG
+    from the best iterate; ``history`` records both phases."""
     t0 = time.perf_counter()
     dinv32, _ = K32.block_jacobi()
-    x, it, _ = pcg(K32, F, dinv32, tol, maxiter, exact=K64, drop=drop)
+    st = {}
+    x, it, _ = pcg(K32, F, dinv32, tol, maxiter, exact=K64, drop=drop, patience=patience, status=st)
+    x = x.clone()
+    hist = [{"mixed_iterations": it, "stalled": st["stalled"]}]
+    outer = 1
+    if st["stalled"]:
+        # the FP32 matrix cannot reach tol on this K: finish with FP64 PCG from
+        # the best iterate (or from zero if none beat it)
+        x0 = st["best_x"] if (st["best_x"] is not None and st["best_rel"] < 1.0) else torch.zeros_like(F)
+        hist[0]["best_rel"] = st["best_rel"]
+        dinv64, _ = K64.block_jacobi()
+        x, it2, _ = pcg(K64, F, dinv64, tol, maxiter, x0=x0)
+        x = x.clone()
+        it += it2
+        outer = 2
+        hist.append({"fp64_iterations": it2})
     _, rel = true_residual(K64, x, F)
     torch.cuda.synchronize()
-    return x.clone(), SolveInfo("mixed", it, 1, rel, time.perf_counter() - t0, [])
+    return x, SolveInfo("mixed", it, outer, rel, time.perf_counter() - t0, hist)

@@ -79,3 +79,23 @@ def test_zero_padding_quirk(cantilever):
     s.Solve_cpu(num_eigen=5, num_block=4, seed=0)
     lam = s.eigenvalues.cpu().numpy()
     assert np.all(np.isfinite(lam)) and np.all(np.diff(lam) >= 0)
+
+
+def test_eigen_solver_ansys_mesh2(gpu):
+    """The reference's Mesh/2.cdb, root sets fixed: the lowest modes against
+    scipy's shift-invert eigsh on the oracle's K and M."""
+    from fem_taichi_amd import Model, Physics
+    from fem_taichi_amd.solve_eigen import Eigen_solver
+    from fixtures import STEEL, load_mesh2
+    Node, Elem, Bounds, Forces = load_mesh2()
+    print("\ncase ansys_mesh2 modes", flush=True)
+    phys = Physics(Node, Elem, Bounds, Forces, name="mesh2")
+    model = Model("mesh2", phys, dict(STEEL), Omegas=(0, 0, 0))
+    pp = O.pre_process(Node, Elem)
+    K = O.model_K_csr(Node, Elem, pp, STEEL["EX"], STEEL["PO"], Bounds)
+    M = O.model_M_csr(Node, Elem, pp, STEEL["DE"])
+    ref = np.sort(sla.eigsh(K.tocsc(), k=6, M=M.tocsr(), sigma=0, which='LM', return_eigenvectors=False))
+    solver = Eigen_solver(model, 0)
+    solver.Solve_cpu(num_eigen=16, num_block=8, seed=2)
+    lam = solver.eigenvalues.cpu().numpy()
+    assert np.allclose(lam[:4], ref[:4], rtol=EIG_TOL, atol=0)

@@ -22,11 +22,23 @@ def rel(a, b):
     return np.linalg.norm(a - b) / nb if nb > 0 else np.linalg.norm(a)
 
 
+def ansys_mesh2():
+    """The reference's own Mesh/2.cdb (unstructured hex, gapped node IDs),
+    root sets fixed, steel, omega = (0, 0, 100): Morton row groups."""
+    from types import SimpleNamespace
+    from fixtures import STEEL, load_mesh2
+    Node, Elem, Bounds, Forces = load_mesh2()
+    return SimpleNamespace(Node=Node, Elem=Elem, Bounds=Bounds, Forces=Forces, Materials=dict(STEEL),
+                           EX=STEEL["EX"], PO=STEEL["PO"], DE=STEEL["DE"], Omegas=(0, 0, 100),
+                           name="ansys_mesh2", node_dims=None)
+
+
 def cases():
     return {
         "box_3x4x5": mesh.structured_block(3, 4, 5, 1.0, 0.0, 0),
         "jit_6x5x7_lognormal": mesh.structured_block(6, 5, 7, 1.0, 0.10, 1, E_sigma=0.3),
         "twisted_10x4x4": mesh.twisted_blade(10, 4, 4, seed=3),
+        "ansys_mesh2": ansys_mesh2(),
     }
 
 
@@ -139,3 +151,19 @@ def test_static_mixed(setup):
     Kref = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
     Fref = O.model_F(g.Node, g.Elem, pp, g.DE, g.Omegas, g.Forces)
     assert O.static_residual(Kref, U, Fref) < 1e-9
+
+
+@pytest.mark.gpu
+def test_static_mixed_fallback(setup):
+    """patience=0 declares the FP32-matrix phase stalled at its first FP64
+    residual replacement; the FP64 PCG that finishes the solve must still
+    reach the tolerance on the oracle's K and F."""
+    from fem_taichi_amd import solvers
+    g, phys, pp, model = setup
+    K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32"))
+    U, info = solvers.solve_mixed(K64, K32, model.F_device(), tol=1e-10, patience=0)
+    assert info.outer == 2 and info.history[0]["stalled"]
+    Kref = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
+    Fref = O.model_F(g.Node, g.Elem, pp, g.DE, g.Omegas, g.Forces)
+    assert O.static_residual(Kref, U.cpu().numpy(), Fref) < 1e-9

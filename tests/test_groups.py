@@ -160,3 +160,24 @@ def test_tasks_reject_crowded_node_pair():
     groups = Gr.build_row_groups(N, 65, ne_ptr, ne_list, gid, 200)
     with pytest.raises(ValueError, match="more than 64"):
         Gr.build_tasks(N, en, sm, re, ne_ptr, ne_list, groups)
+
+
+def test_tasks_ansys_mesh2_morton():
+    """The reference's Mesh/2.cdb through Morton row groups and task lists."""
+    from fixtures import load_mesh2
+    Node, Elem, _, _ = load_mesh2()
+    pp = O.pre_process(Node, Elem)
+    en = O.element_nodes(Elem, pp["Nind"])
+    N = Node.shape[1]
+    ne_ptr, ne_list = incidences(en, N)
+    gid = Gr.node_group_ids(N, xyz=np.asarray(Node[-3:], dtype=np.float64).T)
+    groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 48, co_node=pp["CoNode"])
+    tp, tasks = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    assert np.diff(tp).max() <= 512
+    check_block_lanes(tasks, tp)
+    Coord = O.element_coords(Node, Elem, pp["Nind"])
+    Ke = O.element_K(Coord, 2.0e5, 0.3)
+    Kt, hits = emulate(tasks, tp, groups, Ke, pp["SmNode"], int(pp["SmNode"][-1]))
+    assert np.all(hits == 1)
+    Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
+    assert np.linalg.norm(Kt.reshape(-1) - Kref.reshape(-1)) / np.linalg.norm(Kref) < 1e-14
