@@ -116,7 +116,11 @@ def time_solves(mesh_spec):
     from fem_taichi_amd.learned import solve_learned
     for key, use_net, matrix in (("learned_fcg_f32_matrix", True, "f32"), ("learned_fcg_f64_matrix", True, "f64"),
                                  ("fcg_f64_matrix_bj_only", False, "f64")):
-        _, info = solve_learned(model, K64, F, tol=1e-8, use_net=use_net, matrix=matrix)
+        # the flexible iteration runs to tol unless it diverges or its FP64
+        # residual stalls (no plateau test), so the counts compare the
+        # preconditioners themselves; FP64 PCG finishes only after a stall
+        _, info = solve_learned(model, K64, F, tol=1e-8, use_net=use_net, matrix=matrix, stagnation=None,
+                                maxiter=20000)
         out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual,
                     "fell_back": bool(info.history[0]["fell_back"])}
         print(f"# {key} {json.dumps(out[key])}", flush=True)

@@ -28,7 +28,12 @@ namespace fem {
 namespace {
 
 constexpr int kAsmThreads = 512;
-constexpr int kAStride = 25;     // (e, k) stride of a[], elements of T: 24 used + 1 pad (LDS banks)
+// (e, k) row stride of a[] in elements of T.  FP64: 24 used + 2 pad; 26
+// doubles = 208 B keeps every row 16-byte aligned for 128-bit LDS reads and
+// spreads rows over 16 distinct groups of 4 banks.  FP32: 24 + 1, an odd
+// number of 4-byte banks between rows.
+template <typename T>
+__host__ __device__ constexpr int a_stride() { return sizeof(T) == 8 ? 26 : 25; }
 constexpr int kNone = 0x3FFF;
 
 struct AsmLds {
@@ -50,7 +55,7 @@ __host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bo
   AsmLds L;
   int o = 0;
   L.a_off = o;
-  if (want_k) o += eg * 8 * kAStride * (int)sizeof(T);
+  if (want_k) o += eg * 8 * a_stride<T>() * (int)sizeof(T);
   o = align16(o);
   L.w_off = o;
   if (want_k) o += eg * 8 * (int)sizeof(T);
@@ -84,17 +89,82 @@ __device__ __forceinline__ void gram(const T* __restrict__ s_a, const T* __restr
   const bool has = c != kNone;
   const int cc = has ? c : 0;
   const int le = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
-  const T* aj = s_a + (le * 8 + j) * kAStride;
-  const T* ak = s_a + (le * 8 + k) * kAStride;
+  const T* aj = s_a + (le * 8 + j) * a_stride<T>();
+  const T* ak = s_a + (le * 8 + k) * a_stride<T>();
   const T* w = s_w + le * 8;
+  if constexpr (sizeof(T) == 8) {
+    // two Gauss points per step: 6 doubles of each row = three 128-bit reads
 #pragma unroll 2
-  for (int g = 0; g < 8; ++g) {
-    const T wg = has ? w[g] : T(0);
-    const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
-    const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
-    G[0] = fma(u0, b0, G[0]); G[1] = fma(u0, b1, G[1]); G[2] = fma(u0, b2, G[2]);
-    G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
-    G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
+    for (int gp = 0; gp < 4; ++gp) {
+      const double2* pj = reinterpret_cast<const double2*>(aj + 6 * gp);
+      const double2* pk = reinterpret_cast<const double2*>(ak + 6 * gp);
+      const double2 ww = reinterpret_cast<const double2*>(w)[gp];
+      const double2 j0 = pj[0], j1 = pj[1], j2 = pj[2];
+      const double2 k0 = pk[0], k1 = pk[1], k2 = pk[2];
+      const double wa = has ? ww.x : 0.0, wb = has ? ww.y : 0.0;
+      const double ja[3] = {j0.x, j0.y, j1.x}, jb[3] = {j1.y, j2.x, j2.y};
+      const double ka[3] = {k0.x, k0.y, k1.x}, kb[3] = {k1.y, k2.x, k2.y};
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const double u = wa * ja[p], v = wb * jb[p];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) G[3 * p + q] = fma(v, kb[q], fma(u, ka[q], G[3 * p + q]));
+      }
+    }
+  } else {
+#pragma unroll 2
+    for (int g = 0; g < 8; ++g) {
+      const T wg = has ? w[g] : T(0);
+      const T u0 = wg * aj[3 * g], u1 = wg * aj[3 * g + 1], u2 = wg * aj[3 * g + 2];
+      const T b0 = ak[3 * g], b1 = ak[3 * g + 1], b2 = ak[3 * g + 2];
+      G[0] = fma(u0, b0, G[0]); G[1] = fma(u0, b1, G[1]); G[2] = fma(u0, b2, G[2]);
+      G[3] = fma(u1, b0, G[3]); G[4] = fma(u1, b1, G[4]); G[5] = fma(u1, b2, G[5]);
+      G[6] = fma(u2, b0, G[6]); G[7] = fma(u2, b1, G[7]); G[8] = fma(u2, b2, G[8]);
+    }
+  }
+}
+
+// Lane exchange inside a row of 16 lanes by DPP (VALU, no LDS traffic).
+// CTRL 0xB1 / 0x4E: quad_perm [1,0,3,2] / [2,3,0,1] (partner lane ^1 / ^2);
+// 0x141: row_half_mirror (lane i <-> 7-i of its 8); 0x140: row_mirror
+// (i <-> 15-i of its 16).  After ^1 and ^2 every lane of a quad holds the
+// quad's sum, so pairing with the mirrored lane of the other quad (other
+// half-row) completes the sum just as ^4 (^8) would.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the T adjacent, T-aligned lanes of each block (T <= 32); every
+// lane of the block ends with the same total.  Steps no lane of the wave
+// needs are skipped (wave-uniform test).
+template <typename T, int N>
+__device__ __forceinline__ void block_allsum(T (&G)[N], int Tn) {
+  if (__any(Tn > 1)) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) { const T o = dpp_move<0xB1>(G[q]); if (Tn > 1) G[q] += o; }
+  }
+  if (__any(Tn > 2)) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) { const T o = dpp_move<0x4E>(G[q]); if (Tn > 2) G[q] += o; }
+  }
+  if (__any(Tn > 4)) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) { const T o = dpp_move<0x141>(G[q]); if (Tn > 4) G[q] += o; }
+  }
+  if (__any(Tn > 8)) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) { const T o = dpp_move<0x140>(G[q]); if (Tn > 8) G[q] += o; }
+  }
+  if (__any(Tn > 16)) {
+#pragma unroll
+    for (int q = 0; q < N; ++q) { const T o = __shfl_xor(G[q], 16, 64); if (Tn > 16) G[q] += o; }
   }
 }
 
@@ -107,28 +177,33 @@ __device__ __forceinline__ double mass_term(const double* __restrict__ s_wm, con
   return ms;
 }
 
+template <typename T>
+__device__ __forceinline__ void k_block(const T G[9], T lh, T mh, T v[9]) {
+  const T tr = mh * (G[0] + G[4] + G[8]);
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) v[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ int64_t k_addr(int64_t sm, int co, int m, int idx) {
+  if (LAYOUT == FEM_LAYOUT_BSR) return 9 * (sm + m) + idx;
+  const int p = (idx * 11) >> 5, q = idx - 3 * p;   // idx / 3, idx % 3 for idx < 9
+  return 9 * sm + 3 * (int64_t)p * co + 3 * m + q;
+}
+
+// All 9 values of a block from one lane.
 template <typename T, int LAYOUT>
 __device__ __forceinline__ void store_block(T* __restrict__ k_vals, const int32_t* s_sm, const int32_t* s_co,
                                             int out, const T G[9], T lh, T mh) {
   const int r = out >> 8, m = out & 255;
   const int64_t sm = s_sm[r];
-  const T tr = mh * (G[0] + G[4] + G[8]);
+  const int co = s_co[r];
   T v[9];
+  k_block(G, lh, mh, v);
 #pragma unroll
-  for (int p = 0; p < 3; ++p)
-#pragma unroll
-    for (int q = 0; q < 3; ++q) v[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
-  if (LAYOUT == FEM_LAYOUT_BSR) {
-    T* dst = k_vals + 9 * (sm + m);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) dst[q] = v[q];
-  } else {
-    const int64_t co = s_co[r];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int q = 0; q < 3; ++q) k_vals[9 * sm + 3 * p * co + 3 * m + q] = v[3 * p + q];
-  }
+  for (int idx = 0; idx < 9; ++idx) k_vals[k_addr<LAYOUT>(sm, co, m, idx)] = v[idx];
 }
 
 // 4 waves per SIMD (<= 128 VGPRs): two work-groups of 512 threads per CU.
@@ -215,7 +290,7 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
       for (int k = 0; k < 8; ++k) {
         double dn[3];
         dn_factored(k, fp, fm, dn);
-        T* dst = s_a + (le * 8 + k) * kAStride + 3 * g;
+        T* dst = s_a + (le * 8 + k) * a_stride<T>() + 3 * g;
 #pragma unroll
         for (int a = 0; a < 3; ++a) dst[a] = (T)(dn[0] * Ji[0 * 3 + a] + dn[1] * Ji[1 * 3 + a] + dn[2] * Ji[2 * 3 + a]);
       }
@@ -251,26 +326,16 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
 #pragma unroll
         for (int q = 0; q < 9; ++q) G1[q] += G2[q];
       }
-      for (int sft = 1; sft < 32; sft <<= 1) {
-        if (!__any(sft < Tn)) break;   // wave-uniform: no lane needs a wider butterfly
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          const T other = __shfl_xor(G1[q], sft, 64);
-          if (sft < Tn) G1[q] += other;
-        }
-      }
+      block_allsum(G1, Tn);
       if (valid && (lane & (Tn - 1)) == 0) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outA, G1, lh, mh);
     }
     if (WANT_M) {
-      double m1 = mass_term(s_wm, s_N, c1), m2 = mass_term(s_wm, s_N, c2);
-      if (!dual) m1 += m2;
-      for (int sft = 1; sft < 32; sft <<= 1) {
-        if (!__any(sft < Tn)) break;
-        const double other = __shfl_xor(m1, sft, 64);
-        if (sft < Tn) m1 += other;
-      }
+      double m1[1] = {mass_term(s_wm, s_N, c1)};
+      const double m2 = mass_term(s_wm, s_N, c2);
+      if (!dual) m1[0] += m2;
+      block_allsum(m1, Tn);
       if (valid && (lane & (Tn - 1)) == 0) {
-        m_blocks[(int64_t)s_sm[outA >> 8] + (outA & 255)] = m1;
+        m_blocks[(int64_t)s_sm[outA >> 8] + (outA & 255)] = m1[0];
         if (has_b) m_blocks[(int64_t)s_sm[outB >> 8] + (outB & 255)] = m2;
       }
     }

@@ -237,7 +237,7 @@ class LearnedPreconditioner:
 
 
 def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, drop=1e-2, patience=4,
-              stagnation=25, mmax=4, trace=None):
+              stagnation=100, mmax=4, trace=None):
     """Truncated flexible CG (FCG(m), Notay 2000): each new direction is
     M-orthogonalised (in the K inner product) against the last ``mmax``
     directions, which keeps CG-like convergence with a nonlinear
@@ -248,7 +248,9 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
     FP64 block-Jacobi PCG (from its best iterate, or from zero; with its own
     ``maxiter``) when ``patience`` consecutive replacements fail to halve the
     best FP64 residual, when the recursive residual sets no new low (by 10%)
-    in ``stagnation`` consecutive checks, or when it turns non-finite.
+    in ``stagnation`` consecutive checks (None: no such test; CG-type
+    residual norms can plateau for long), when it grows 1e4-fold above its
+    lowest value, or when it turns non-finite.
     Returns (x, info dict)."""
     from . import solvers
     st = stream_ptr()
@@ -295,8 +297,10 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
             low_rr, flat = rr, 0
         else:
             flat += 1
-            if flat >= stagnation:
+            if stagnation is not None and flat >= stagnation:
                 break
+        if rr > 1e8 * low_rr:           # the recursion diverges
+            break
         rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
         if rel < tol or rr < drop * drop * rr_anchor:
             r = F - K64.matvec(x)

@@ -81,7 +81,7 @@ class PcgWorkspace:
 def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxiter: int,
         ws: PcgWorkspace | None = None, check_every: int = 16, x0: torch.Tensor | None = None,
         exact: BsrOperator | None = None, drop: float = 1e-2, trace: list | None = None,
-        patience: int = 4, status: dict | None = None):
+        patience: int = 4, gain: float = 0.1, status: dict | None = None):
     """Block-Jacobi PCG to ||r||/||b|| < tol.
 
     ``op`` supplies the matrix used inside the iteration.  When ``exact`` (an
@@ -97,10 +97,14 @@ def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxite
     The FP32 matrix differs from the FP64 one by ~6e-8 per entry; when the
     conditioning of K makes that perturbation comparable to its smallest
     eigenvalues the FP64 residual stops falling however long the recursion
-    runs.  In mixed mode the iteration therefore stops once ``patience``
-    consecutive replacements fail to reduce the best FP64 residual norm by 2x
-    (or it turns non-finite); ``status`` (a dict, if given) then receives
-    "stalled": True and "best_x", the iterate with the smallest FP64 residual.
+    runs, or the recursion diverges.  In mixed mode a replacement also
+    happens when the recursive residual grows by 1/``drop`` or turns
+    non-finite, and the iteration stops once ``patience`` consecutive
+    replacements fail to reduce the best FP64 residual norm by the factor
+    ``gain`` (a sound FP32 matrix gains ~``drop`` per replacement; 0.1 leaves
+    a 10x margin), or it turns non-finite; ``status`` (a dict, if given) then
+    receives "stalled": True and "best_x", the iterate with the smallest
+    FP64 residual.
     Returns (x, iterations, relative residual).
     """
     N = op.mesh.num_node
@@ -142,11 +146,14 @@ def pcg(op: BsrOperator, b: torch.Tensor, dinv: torch.Tensor, tol: float, maxite
         status["stalled"] = False
     while True:
         rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
-        if mixed and it > 0 and (rel < tol or rr < drop * drop * rr_anchor):
+        # replace on claimed convergence, on a drop by ``drop`` in norm and on a
+        # rise by 1/drop (an FP32 recursion that diverges never drops)
+        if mixed and it > 0 and (rel < tol or rr < drop * drop * rr_anchor or not math.isfinite(rr)
+                                 or rr * drop * drop > rr_anchor):
             rr, bb = replace(rr)
             rr_anchor = rr
             rel = (rr / bb) ** 0.5 if bb > 0 else 0.0
-            if math.isfinite(rr) and rr < 0.25 * best_rr:
+            if math.isfinite(rr) and rr < gain * gain * best_rr:
                 best_rr, stalls = rr, 0
                 if status is not None:
                     best_x = ws.x.clone()
@@ -187,7 +194,7 @@ def solve_pcg64(K64: BsrOperator, F: torch.Tensor, tol=1e-8, maxiter=200000):
 
 
 def solve_mixed(K64: BsrOperator, K32: BsrOperator, F: torch.Tensor, tol=1e-8, drop=1e-2,
-                maxiter=200000, patience=4):
+                maxiter=200000, patience=2):
     """PCG whose SpMV streams the FP32 copy of K while vectors, dot products
     and the block-Jacobi application stay FP64; the residual is replaced by
     the FP64 one on the FP64 K (reliable updates), so the stopping test is the

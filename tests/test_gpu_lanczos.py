@@ -95,7 +95,11 @@ def test_eigen_solver_ansys_mesh2(gpu):
     K = O.model_K_csr(Node, Elem, pp, STEEL["EX"], STEEL["PO"], Bounds)
     M = O.model_M_csr(Node, Elem, pp, STEEL["DE"])
     ref = np.sort(sla.eigsh(K.tocsc(), k=6, M=M.tocsr(), sigma=0, which='LM', return_eigenvectors=False))
+    # the lowest modes are clustered (1.839644e5, 1.839649e5 twice, 1.839661e5
+    # twice): 10 block steps separate them (the oracle's Lanczos with exact
+    # solves agrees with eigsh to 1e-13 at these settings, 1e-6 at 16 / 8)
     solver = Eigen_solver(model, 0)
-    solver.Solve_cpu(num_eigen=16, num_block=8, seed=2)
+    solver.Solve_cpu(num_eigen=40, num_block=8, seed=2)
     lam = solver.eigenvalues.cpu().numpy()
-    assert np.allclose(lam[:4], ref[:4], rtol=EIG_TOL, atol=0)
+    print("lambda", lam[:6], "eigsh", ref, flush=True)
+    assert np.allclose(lam[:5], ref[:5], rtol=EIG_TOL, atol=0)

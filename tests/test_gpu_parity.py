@@ -33,12 +33,26 @@ def ansys_mesh2():
                            name="ansys_mesh2", node_dims=None)
 
 
+def crowded():
+    """Two hexes duplicated 20 and 26 times (per-copy modulus): node pairs
+    with 20-46 contributions drive the 16- and 32-lane block sums."""
+    from types import SimpleNamespace
+    g = mesh.structured_block(2, 1, 1, 1.0, 0.1, 5)
+    elem = np.concatenate([np.repeat(g.Elem[-8:, :1], 20, axis=1), np.repeat(g.Elem[-8:, 1:2], 26, axis=1)],
+                          axis=1)
+    E = 2.0e5 * np.exp(0.3 * np.random.default_rng(4).standard_normal(elem.shape[1]))
+    mats = {"EX": E, "PO": 0.3, "DE": 7.8e-9}
+    return SimpleNamespace(Node=g.Node, Elem=elem, Bounds=g.Bounds, Forces=g.Forces, Materials=mats, EX=E,
+                           PO=0.3, DE=7.8e-9, Omegas=(0, 0, 100), name="crowded", node_dims=None)
+
+
 def cases():
     return {
         "box_3x4x5": mesh.structured_block(3, 4, 5, 1.0, 0.0, 0),
         "jit_6x5x7_lognormal": mesh.structured_block(6, 5, 7, 1.0, 0.10, 1, E_sigma=0.3),
         "twisted_10x4x4": mesh.twisted_blade(10, 4, 4, seed=3),
         "ansys_mesh2": ansys_mesh2(),
+        "crowded_46": crowded(),
     }
 
 

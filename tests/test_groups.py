@@ -65,16 +65,17 @@ def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512):
                     outB.append(-1 if ob == 0xFFFF else ob)
                     dual.append(d)
                     valid.append(ok)
-                s = 1
-                while s < 32:
+                # the kernel's steps: lane ^1, ^2 (quad_perm), ^7 (row_half_mirror),
+                # ^15 (row_mirror), ^16; step k runs for lanes with T > 2^k and
+                # is skipped when no lane of the wave needs it
+                for s, xor in ((1, 1), (2, 2), (4, 7), (8, 15), (16, 16)):
                     if not any(s < x for x in T):
-                        break
+                        continue
                     new = list(val)
                     for li in range(wave):
                         if s < T[li]:
-                            new[li] = val[li] + val[li ^ s]
+                            new[li] = val[li] + val[li ^ xor]
                     val = new
-                    s <<= 1
                 for li in range(wave):
                     if not valid[li] or (li & (T[li] - 1)) != 0:
                         continue
@@ -99,6 +100,17 @@ def check_block_lanes(tasks, tp):
             assert (t - tp[g]) % Tn == 0
             assert np.all(outa[t:t + Tn] == outa[t])
             t += Tn
+
+
+def crowded_mesh():
+    """Two neighbouring hexes, duplicated 20 and 26 times with a lognormal
+    modulus per copy: node pairs with 20, 26 and 46 contributions (T = 16, 16,
+    32), the wide butterfly steps structured meshes never reach."""
+    g = mesh.structured_block(2, 1, 1, 1.0, 0.1, 5)
+    elem = np.concatenate([np.repeat(g.Elem[-8:, :1], 20, axis=1), np.repeat(g.Elem[-8:, 1:2], 26, axis=1)],
+                          axis=1)
+    E = 2.0e5 * np.exp(0.3 * np.random.default_rng(4).standard_normal(elem.shape[1]))
+    return g, elem, E
 
 
 CASES = [
@@ -177,6 +189,26 @@ def test_tasks_ansys_mesh2_morton():
     check_block_lanes(tasks, tp)
     Coord = O.element_coords(Node, Elem, pp["Nind"])
     Ke = O.element_K(Coord, 2.0e5, 0.3)
+    Kt, hits = emulate(tasks, tp, groups, Ke, pp["SmNode"], int(pp["SmNode"][-1]))
+    assert np.all(hits == 1)
+    Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
+    assert np.linalg.norm(Kt.reshape(-1) - Kref.reshape(-1)) / np.linalg.norm(Kref) < 1e-14
+
+
+def test_tasks_crowded_wide_butterfly():
+    g, elem, E = crowded_mesh()
+    Node = g.Node
+    pp = O.pre_process(Node, elem)
+    en = O.element_nodes(elem, pp["Nind"])
+    N = Node.shape[1]
+    ne_ptr, ne_list = incidences(en, N)
+    gid = Gr.node_group_ids(N, xyz=np.asarray(Node[-3:], dtype=np.float64).T)
+    groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 48, co_node=pp["CoNode"])
+    tp, tasks = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    lt = (tasks[:, 1].view(np.uint32) >> 28) & 7
+    assert lt.max() == 5 and 4 in set(lt.tolist())      # T = 32 and T = 16 blocks present
+    check_block_lanes(tasks, tp)
+    Ke = O.element_K(O.element_coords(Node, elem, pp["Nind"]), E, 0.3)
     Kt, hits = emulate(tasks, tp, groups, Ke, pp["SmNode"], int(pp["SmNode"][-1]))
     assert np.all(hits == 1)
     Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
