@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time the static solver forms")
     ap.add_argument("--solve-mesh", default="400x50x50")
+    ap.add_argument("--modal", default=None, metavar="NXxNYxNZ:MODES:BLOCK",
+                    help="also time block Lanczos (Eigen_solver) on a twisted blade, e.g. 400x50x50:20:8")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (the known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -128,6 +130,41 @@ def time_solves(mesh_spec):
     return out
 
 
+def time_modal(spec):
+    """Shift-invert block Lanczos on a twisted blade: wall time, inner block
+    PCG iterations, and the shift-invert residual lam*||K^-1 M x - x/lam||/||x||
+    of each returned mode (K^-1 by a tight FP64 block PCG)."""
+    import torch
+    from fem_taichi_amd import Model, Physics, mesh
+    from fem_taichi_amd.solve_eigen import Eigen_solver
+    dims, modes, block = spec.split(":")
+    nx, ny, nz = (int(v) for v in dims.split("x"))
+    modes, block = int(modes), int(block)
+    g = mesh.twisted_blade(nx, ny, nz)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=(0, 0, 0))
+    solver = Eigen_solver(model, 0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    lam = solver.eigenvalues
+    ops = solver.ops
+    Xd = torch.as_tensor(X[:, :min(8, modes)]).cuda()
+    res = []
+    MX = ops.m_apply(Xd)
+    Y = ops.solve(MX, 1e-12)
+    for c in range(Xd.shape[1]):
+        lc = float(lam[c])
+        res.append(float(lc * torch.linalg.vector_norm(Y[:, c] - Xd[:, c] / lc) / torch.linalg.vector_norm(Xd[:, c])))
+    out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "modes": modes, "block": block,
+           "seconds": secs, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
+           "shift_invert_residual_first8": res}
+    print(f"# modal {json.dumps(out)}", flush=True)
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -203,6 +240,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(g, phys)
     if args.solve:
         result["static_solve"] = time_solves(args.solve_mesh)
+    if args.modal:
+        result["modal"] = time_modal(args.modal)
     if args.copy_probe:
         copy_probe()
     print(json.dumps(result), flush=True)

@@ -15,11 +15,15 @@ namespace {
 
 constexpr int kT = 256;
 constexpr int kGridB = 512;
+// Threads per work-group of the kernels that stream the n x nb blocks: with
+// the fixed grid of kGridB work-groups, 1024 threads fill a CU.
+constexpr int kTS = 1024;
 
-// Per-column sum over the block: s[tid] holds values whose column is tid % nb;
-// result for column c < nb ends in s[c].
+// Per-column sum over the block of NT threads: s[tid] holds values whose
+// column is tid % nb; result for column c < nb ends in s[c].
+template <int NT = kT>
 __device__ void col_reduce(double* s, int nb) {
-  for (int w = kT / 2; w >= nb; w >>= 1) {
+  for (int w = NT / 2; w >= nb; w >>= 1) {
     __syncthreads();
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
   }
@@ -28,26 +32,29 @@ __device__ void col_reduce(double* s, int nb) {
 
 // out[c] = sum over blocks of part[b * nb + c] for every column c < nb, in
 // the same order in every block (out is shared memory).
+template <int NT = kT>
 __device__ void col_partials(const double* __restrict__ part, int nb, double* s, double* out) {
-  // each thread sums blocks b = tid / nb, tid / nb + kT / nb, ... for column tid % nb
+  // each thread sums blocks b = tid / nb, tid / nb + NT / nb, ... for column tid % nb
   const int cc = threadIdx.x % nb;
   double v = 0.0;
-  for (int b = threadIdx.x / nb; b < kGridB; b += kT / nb) v += part[b * nb + cc];
+  for (int b = threadIdx.x / nb; b < kGridB; b += NT / nb) v += part[b * nb + cc];
   s[threadIdx.x] = v;
-  col_reduce(s, nb);
+  col_reduce<NT>(s, nb);
   if ((int)threadIdx.x < nb) out[threadIdx.x] = s[threadIdx.x];
   __syncthreads();
 }
 
 // Y = A X with per-column partial X.Y (A: BSR 3x3 f64).  Thread = (row i, c).
-__global__ void __launch_bounds__(kT)
+// kTS threads per work-group on the fixed grid of kGridB: the gathers of X
+// (3 nb values per block) need the memory parallelism of a full CU.
+__global__ void __launch_bounds__(kTS)
 mspmm_dot_kernel(fem_mesh m, const double* __restrict__ vals, const double* __restrict__ X,
                  double* __restrict__ Y, int nb, double* __restrict__ part, double* state) {
-  __shared__ double s[kT];
+  __shared__ double s[kTS];
   const int c = threadIdx.x % nb;
   double d = 0.0;
   const int64_t total = m.num_node * nb;
-  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+  for (int64_t t = (int64_t)blockIdx.x * kTS + threadIdx.x; t < total; t += (int64_t)gridDim.x * kTS) {
     const int64_t i = t / nb;
     const int64_t sm = m.sm_node[i];
     const int co = m.co_node[i];
@@ -67,7 +74,7 @@ mspmm_dot_kernel(fem_mesh m, const double* __restrict__ vals, const double* __re
   }
   if (part) {
     s[threadIdx.x] = d;
-    col_reduce(s, nb);
+    col_reduce<kTS>(s, nb);
     if ((int)threadIdx.x < nb) part[blockIdx.x * nb + threadIdx.x] = s[threadIdx.x];
     // move rz_next (row 1) into rz (row 0) for the update kernel
     if (blockIdx.x == 0 && (int)threadIdx.x < nb) state[threadIdx.x] = state[nb + threadIdx.x];
@@ -122,14 +129,14 @@ __global__ void mpcg_start_finalize(const double* __restrict__ part, int nb, dou
   }
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kTS)
 mpcg_update_kernel(int64_t N, int nb, double* __restrict__ X, double* __restrict__ R, double* __restrict__ Z,
                    const double* __restrict__ P, const double* __restrict__ Q, const double* __restrict__ dinv,
                    const double* __restrict__ part_pq, double* __restrict__ part_out, double* state) {
-  __shared__ double s[kT];
+  __shared__ double s[kTS];
   __shared__ double pq_s[64];
   __shared__ double alpha_s[64];
-  col_partials(part_pq, nb, s, pq_s);
+  col_partials<kTS>(part_pq, nb, s, pq_s);
   if ((int)threadIdx.x < nb) {
     const int c = threadIdx.x;
     const double pq = pq_s[c];
@@ -145,7 +152,7 @@ mpcg_update_kernel(int64_t N, int nb, double* __restrict__ X, double* __restrict
   const double a = alpha_s[c];
   double rz = 0, rr = 0;
   const int64_t total = N * nb;
-  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT) {
+  for (int64_t t = (int64_t)blockIdx.x * kTS + threadIdx.x; t < total; t += (int64_t)gridDim.x * kTS) {
     const int64_t i = t / nb;
     double r[3], z[3];
     for (int q = 0; q < 3; ++q) {
@@ -163,23 +170,23 @@ mpcg_update_kernel(int64_t N, int nb, double* __restrict__ X, double* __restrict
     }
   }
   s[threadIdx.x] = rz;
-  col_reduce(s, nb);
+  col_reduce<kTS>(s, nb);
   if ((int)threadIdx.x < nb) part_out[blockIdx.x * nb + threadIdx.x] = s[threadIdx.x];
   __syncthreads();
   s[threadIdx.x] = rr;
-  col_reduce(s, nb);
+  col_reduce<kTS>(s, nb);
   if ((int)threadIdx.x < nb) part_out[(kGridB + blockIdx.x) * nb + threadIdx.x] = s[threadIdx.x];
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kTS)
 mpcg_direction_kernel(int64_t N, int nb, const double* __restrict__ Z, double* __restrict__ P,
                       const double* __restrict__ part, double* state) {
-  __shared__ double s[kT];
+  __shared__ double s[kTS];
   __shared__ double rz_s[64];
   __shared__ double rr_s[64];
   __shared__ double beta_s[64];
-  col_partials(part, nb, s, rz_s);
-  if (blockIdx.x == 0) col_partials(part + kGridB * nb, nb, s, rr_s);
+  col_partials<kTS>(part, nb, s, rz_s);
+  if (blockIdx.x == 0) col_partials<kTS>(part + kGridB * nb, nb, s, rr_s);
   if ((int)threadIdx.x < nb) {
     const int c = threadIdx.x;
     const double rz = state[c];
@@ -195,7 +202,7 @@ mpcg_direction_kernel(int64_t N, int nb, const double* __restrict__ Z, double* _
   const int c = threadIdx.x % nb;
   const double b = beta_s[c];
   const int64_t total = 3 * N * nb;
-  for (int64_t t = (int64_t)blockIdx.x * kT + threadIdx.x; t < total; t += (int64_t)gridDim.x * kT)
+  for (int64_t t = (int64_t)blockIdx.x * kTS + threadIdx.x; t < total; t += (int64_t)gridDim.x * kTS)
     P[t] = Z[t] + b * P[t];
 }
 
@@ -223,7 +230,7 @@ extern "C" int fem_mpcg_start(const fem_mesh* mesh, const double* vals, const do
   hipStream_t st = as_stream(stream);
   double* part = reinterpret_cast<double*>(work);
   const int nb = (int)nrhs;
-  mspmm_dot_kernel<<<kGridB, kT, 0, st>>>(*mesh, vals, X, Z, nb, nullptr, nullptr);
+  mspmm_dot_kernel<<<kGridB, kTS, 0, st>>>(*mesh, vals, X, Z, nb, nullptr, nullptr);
   mpcg_start_kernel<<<kGridB, kT, 0, st>>>(mesh->num_node, nb, B, Z, dinv, R, Z, P, part);
   mpcg_start_finalize<<<1, kT, 0, st>>>(part, nb, state);
   FEM_LAUNCH_CHECK();
@@ -242,9 +249,9 @@ extern "C" int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const 
   double* part = reinterpret_cast<double*>(work);
   const int nb = (int)nrhs;
   const int64_t N = mesh->num_node;
-  mspmm_dot_kernel<<<kGridB, kT, 0, st>>>(*mesh, vals, P, Q, nb, part, state);
-  mpcg_update_kernel<<<kGridB, kT, 0, st>>>(N, nb, X, R, Z, P, Q, dinv, part, part + kGridB * nb, state);
-  mpcg_direction_kernel<<<kGridB, kT, 0, st>>>(N, nb, Z, P, part + kGridB * nb, state);
+  mspmm_dot_kernel<<<kGridB, kTS, 0, st>>>(*mesh, vals, P, Q, nb, part, state);
+  mpcg_update_kernel<<<kGridB, kTS, 0, st>>>(N, nb, X, R, Z, P, Q, dinv, part, part + kGridB * nb, state);
+  mpcg_direction_kernel<<<kGridB, kTS, 0, st>>>(N, nb, Z, P, part + kGridB * nb, state);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
