@@ -432,6 +432,244 @@ __global__ void penalty_kernel(fem_mesh m, const int32_t* __restrict__ bnode,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Large-displacement matrices (UnitK, FEM_model_bsr.py:476-598): the tangent
+// KT and the matrix KM of the residual KM U - F, at the displacement u.
+// Closed form of the reference's B / BN products, per Gauss point, with
+// F = I + du/dx, Fh = I + du/dx / 2, A = F a, H = Fh a (a: physical shape
+// gradients), lambda = E lhat, mu = E mhat:
+//   KT_jk += detJ [lambda A_j A_k^T + mu A_k A_j^T + mu (a_j.a_k) F F^T] + detJ (a_j^T s a_k) I
+//   KM_jk += detJ [lambda A_j H_k^T + mu A_k H_j^T + mu (a_j.a_k) F Fh^T]
+// where F Fh^T = (F + F F^T) / 2 and s is the element stress accumulated
+// over the Gauss points visited so far (the reference sums S = lambda tr(E) I
+// + 2 mu E, E the Green strain, into one element field inside its Gauss
+// loop), taken as an inclusive scan over the element's 8 Gauss lanes.  Same
+// row groups / task lists / lane butterfly as the linear kernel; FP64 only.
+// ---------------------------------------------------------------------------
+constexpr int kKnThreads = 256;
+constexpr int kKnG = 24;   // per (e, g): wl, wm, F[9], FFt[6], st[6] (xx yy zz xy xz yz), pad
+
+struct KnLds {
+  int a_off, g_off, sm_off, co_off, total;
+};
+
+__host__ __device__ inline KnLds kn_lds_layout(int eg, int rg) {
+  KnLds L;
+  int o = 0;
+  L.a_off = o;
+  o += eg * 8 * a_stride<double>() * 8;
+  L.g_off = o;
+  o += eg * 8 * kKnG * 8;
+  L.sm_off = o;
+  o += rg * 4;
+  L.co_off = o;
+  o += rg * 4;
+  L.total = align16(o);
+  return L;
+}
+
+__device__ __forceinline__ void kn_contrib(const double* __restrict__ s_a, const double* __restrict__ s_g, int c,
+                                           double KT[9], double KM[9]) {
+  const bool has = c != kNone;
+  const int cc = has ? c : 0;
+  const int le = cc >> 6, j = (cc >> 3) & 7, k = cc & 7;
+  const double* aj = s_a + (le * 8 + j) * a_stride<double>();
+  const double* ak = s_a + (le * 8 + k) * a_stride<double>();
+  const double on = has ? 1.0 : 0.0;
+#pragma unroll 1
+  for (int g = 0; g < 8; ++g) {
+    const double* q = s_g + (le * 8 + g) * kKnG;
+    const double wl = on * q[0], wm = on * q[1];
+    const double F[9] = {q[2], q[3], q[4], q[5], q[6], q[7], q[8], q[9], q[10]};
+    const double FF[9] = {q[11], q[14], q[15], q[14], q[12], q[16], q[15], q[16], q[13]};
+    const double st[9] = {q[17], q[20], q[21], q[20], q[18], q[22], q[21], q[22], q[19]};
+    const double a1[3] = {aj[3 * g], aj[3 * g + 1], aj[3 * g + 2]};
+    const double a2[3] = {ak[3 * g], ak[3 * g + 1], ak[3 * g + 2]};
+    double A1[3], A2[3], H1[3], H2[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      A1[r] = F[3 * r] * a1[0] + F[3 * r + 1] * a1[1] + F[3 * r + 2] * a1[2];
+      A2[r] = F[3 * r] * a2[0] + F[3 * r + 1] * a2[1] + F[3 * r + 2] * a2[2];
+      H1[r] = 0.5 * (a1[r] + A1[r]);
+      H2[r] = 0.5 * (a2[r] + A2[r]);
+    }
+    const double d = a1[0] * a2[0] + a1[1] * a2[1] + a1[2] * a2[2];
+    double gsc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) gsc += a1[r] * (st[3 * r] * a2[0] + st[3 * r + 1] * a2[1] + st[3 * r + 2] * a2[2]);
+    gsc *= on;
+    const double wmd = wm * d;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int n = 0; n < 3; ++n) {
+        KT[3 * r + n] += wl * A1[r] * A2[n] + wm * A2[r] * A1[n] + wmd * FF[3 * r + n] + (r == n ? gsc : 0.0);
+        KM[3 * r + n] += wl * A1[r] * H2[n] + wm * A2[r] * H1[n] + wmd * 0.5 * (F[3 * r + n] + FF[3 * r + n]);
+      }
+  }
+}
+
+template <int LAYOUT>
+__device__ __forceinline__ void store_raw(double* __restrict__ vals, const int32_t* s_sm, const int32_t* s_co, int out,
+                                          const double v[9]) {
+  const int r = out >> 8, m = out & 255;
+  const int64_t sm = s_sm[r];
+  const int co = s_co[r];
+#pragma unroll
+  for (int idx = 0; idx < 9; ++idx) vals[k_addr<LAYOUT>(sm, co, m, idx)] = v[idx];
+}
+
+template <int LAYOUT>
+__global__ void __launch_bounds__(kKnThreads)
+hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar, double lhat, double mhat,
+               const double* __restrict__ u, double* __restrict__ kt, double* __restrict__ km, int eg, int rg) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const KnLds L = kn_lds_layout(eg, rg);
+  double* s_a = reinterpret_cast<double*>(smem + L.a_off);
+  double* s_g = reinterpret_cast<double*>(smem + L.g_off);
+  int32_t* s_sm = reinterpret_cast<int32_t*>(smem + L.sm_off);
+  int32_t* s_co = reinterpret_cast<int32_t*>(smem + L.co_off);
+  const int grp = xcd_chunked(blockIdx.x, gridDim.x);
+  const int e0 = m.grp_elem_ptr[grp];
+  const int ne = m.grp_elem_ptr[grp + 1] - e0;
+  const int r0 = m.grp_row_ptr[grp];
+  const int nr = m.grp_row_ptr[grp + 1] - r0;
+  const int t0 = m.grp_task_ptr[grp];
+  const int nt = m.grp_task_ptr[grp + 1] - t0;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int le = tid >> 3, g = tid & 7;
+  const bool p1 = tid < ne * 8;   // ne * 8 <= kKnThreads (checked on the host)
+  for (int r = tid; r < nr; r += kKnThreads) {
+    const int32_t i = m.grp_rows[r0 + r];
+    s_sm[r] = m.sm_node[i];
+    s_co[r] = m.co_node[i];
+  }
+  // ---- phase 1
+  double S[6] = {0, 0, 0, 0, 0, 0};
+  double det = 0.0, F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, Ee = young_scalar;
+  if (p1) {
+    const int4* en = reinterpret_cast<const int4*>(m.grp_enodes + (int64_t)(e0 + le) * 8);
+    const int4 na = en[0], nb = en[1];
+    const int32_t nodes[8] = {na.x, na.y, na.z, na.w, nb.x, nb.y, nb.z, nb.w};
+    if (young) Ee = young[m.grp_elems[e0 + le]];
+    double fp[3], fm[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      fp[a] = 1.0 + gauss_sign(g, a) * kGS;
+      fm[a] = 1.0 - gauss_sign(g, a) * kGS;
+    }
+    const int64_t n0 = nodes[0];
+    const double x00 = m.coords[n0 * 3], x01 = m.coords[n0 * 3 + 1], x02 = m.coords[n0 * 3 + 2];
+    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 1; k < 8; ++k) {
+      const int64_t nk = nodes[k];
+      const double xr[3] = {m.coords[nk * 3] - x00, m.coords[nk * 3 + 1] - x01, m.coords[nk * 3 + 2] - x02};
+      double dn[3];
+      dn_factored(k, fp, fm, dn);
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr[a], dn[d], J[a * 3 + d]);
+    }
+    double Ji[9];
+    det = inv3(J, Ji);
+    double T[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // T[r][p] = du_r / dx_p
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      double dn[3];
+      dn_factored(k, fp, fm, dn);
+      double a[3];
+#pragma unroll
+      for (int p = 0; p < 3; ++p) a[p] = dn[0] * Ji[0 * 3 + p] + dn[1] * Ji[1 * 3 + p] + dn[2] * Ji[2 * 3 + p];
+      double* dst = s_a + (le * 8 + k) * a_stride<double>() + 3 * g;
+      dst[0] = a[0];
+      dst[1] = a[1];
+      dst[2] = a[2];
+      const int64_t nk = nodes[k];
+      const double uk[3] = {u[nk * 3], u[nk * 3 + 1], u[nk * 3 + 2]};
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) T[3 * r + p] = fma(uk[r], a[p], T[3 * r + p]);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) F[q] = T[q] + ((q % 4 == 0) ? 1.0 : 0.0);
+    // Green strain E = (F^T F - I) / 2 and the stress lambda tr(E) I + 2 mu E
+    double Eg[9];
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        Eg[3 * p + q] = 0.5 * (F[p] * F[q] + F[3 + p] * F[3 + q] + F[6 + p] * F[6 + q] - (p == q ? 1.0 : 0.0));
+    const double lam = Ee * lhat, mu = Ee * mhat;
+    const double tr = Eg[0] + Eg[4] + Eg[8];
+    S[0] = lam * tr + 2 * mu * Eg[0];
+    S[1] = lam * tr + 2 * mu * Eg[4];
+    S[2] = lam * tr + 2 * mu * Eg[8];
+    S[3] = 2 * mu * Eg[1];
+    S[4] = 2 * mu * Eg[2];
+    S[5] = 2 * mu * Eg[5];
+  }
+  // inclusive scan of the stress over the element's Gauss points (8 lanes)
+#pragma unroll
+  for (int off = 1; off < 8; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double t = __shfl_up(S[c], off, 8);
+      if (g >= off) S[c] += t;
+    }
+  }
+  if (p1) {
+    double* q = s_g + (le * 8 + g) * kKnG;
+    q[0] = Ee * det * lhat;
+    q[1] = Ee * det * mhat;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) q[2 + r] = F[r];
+    // F F^T: xx yy zz xy xz yz
+    q[11] = F[0] * F[0] + F[1] * F[1] + F[2] * F[2];
+    q[12] = F[3] * F[3] + F[4] * F[4] + F[5] * F[5];
+    q[13] = F[6] * F[6] + F[7] * F[7] + F[8] * F[8];
+    q[14] = F[0] * F[3] + F[1] * F[4] + F[2] * F[5];
+    q[15] = F[0] * F[6] + F[1] * F[7] + F[2] * F[8];
+    q[16] = F[3] * F[6] + F[4] * F[7] + F[5] * F[8];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) q[17 + c] = det * S[c];
+  }
+  __syncthreads();
+  // ---- phase 2
+  for (int base = 0; base < nt; base += kKnThreads) {
+    const int t = base + tid;
+    const bool valid = t < nt;
+    const int2 task = valid ? reinterpret_cast<const int2*>(m.tasks)[t0 + t]
+                            : make_int2((int)0xFFFF0000u, kNone | (kNone << 14));
+    const int outA = task.x & 0xFFFF, outB = (task.x >> 16) & 0xFFFF;
+    const uint32_t w2 = (uint32_t)task.y;
+    const int c1 = (int)(w2 & 0x3FFF), c2 = (int)((w2 >> 14) & 0x3FFF);
+    const bool dual = (w2 >> 31) & 1u;
+    const int Tn = dual ? 1 : (1 << ((w2 >> 28) & 7u));
+    const bool has_b = dual && outB != 0xFFFF;
+    double acc[18];
+#pragma unroll
+    for (int q = 0; q < 18; ++q) acc[q] = 0.0;
+    kn_contrib(s_a, s_g, c1, acc, acc + 9);
+    if (dual) {   // a dual task's first block is complete: store it, then reuse acc
+      if (valid) {
+        store_raw<LAYOUT>(kt, s_sm, s_co, outA, acc);
+        store_raw<LAYOUT>(km, s_sm, s_co, outA, acc + 9);
+      }
+#pragma unroll
+      for (int q = 0; q < 18; ++q) acc[q] = 0.0;
+    }
+    kn_contrib(s_a, s_g, c2, acc, acc + 9);
+    block_allsum(acc, Tn);
+    if (valid && (lane & (Tn - 1)) == 0 && (!dual || has_b)) {
+      store_raw<LAYOUT>(kt, s_sm, s_co, dual ? outB : outA, acc);
+      store_raw<LAYOUT>(km, s_sm, s_co, dual ? outB : outA, acc + 9);
+    }
+  }
+}
+
 int check_mesh(const fem_mesh* m, bool need_groups) {
   FEM_REQUIRE(m != nullptr, FEM_E_INVALID, "mesh is NULL");
   FEM_REQUIRE(m->num_node >= 0 && m->num_elem >= 0 && m->nnzb >= 0, FEM_E_INVALID, "negative size");
@@ -492,6 +730,35 @@ extern "C" int fem_hex8_assemble(const fem_mesh* mesh, const double* young, doub
              : launch_asm<float, true, false, 1>(m, young, young_scalar, nu, rho, k_vals, m_blocks, st);
   }
   return launch_asm<double, false, true, 0>(m, young, young_scalar, nu, rho, nullptr, m_blocks, st);
+}
+
+extern "C" int64_t fem_hex8_kn_lds_bytes(int32_t max_group_elems, int32_t max_group_rows) {
+  return kn_lds_layout(max_group_elems, max_group_rows).total;
+}
+
+extern "C" int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
+                                    const double* u, double* kt_vals, double* km_vals, int k_layout, void* stream) {
+  int rc = check_mesh(mesh, true);
+  if (rc) return rc;
+  FEM_REQUIRE(u && kt_vals && km_vals, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(k_layout == FEM_LAYOUT_BSR || k_layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad k_layout");
+  FEM_REQUIRE(nu > -1.0 && nu < 0.5, FEM_E_INVALID, "Poisson ratio %g outside (-1, 0.5)", nu);
+  FEM_REQUIRE(mesh->max_group_elems * 8 <= kKnThreads, FEM_E_LIMIT,
+              "large-displacement assembly needs row groups of at most %d elements", kKnThreads / 8);
+  if (mesh->num_groups == 0) return FEM_OK;
+  const int eg = mesh->max_group_elems, rg = mesh->max_group_rows;
+  const KnLds L = kn_lds_layout(eg, rg);
+  FEM_REQUIRE(L.total <= 160 * 1024, FEM_E_LIMIT, "row group needs %d B of LDS (> 160 KB)", L.total);
+  const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
+  const double mhat = 1.0 / (2 * (1 + nu));
+  hipStream_t st = as_stream(stream);
+  auto kern = k_layout == FEM_LAYOUT_BSR ? hex8_kn_kernel<0> : hex8_kn_kernel<1>;
+  FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
+  hipLaunchKernelGGL(kern, dim3((unsigned)mesh->num_groups), dim3(kKnThreads), L.total, st, *mesh, young,
+                     young_scalar, lhat, mhat, u, kt_vals, km_vals, eg, rg);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
 }
 
 extern "C" int fem_csr_index_K(const fem_mesh* mesh, int32_t* row_k, int32_t* col_k, void* stream) {

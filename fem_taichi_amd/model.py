@@ -147,16 +147,57 @@ class Model:
         return self._cache["F"]
 
     # ---------------------------------------------------------------- reference API
-    def Get_MatrixKL(self):
-        """FEM_model_csr.py:118-127: penalised stiffness as scipy CSR (3N x 3N)."""
-        vals = self.assemble_K("f64", "csr")
+    # ---- large displacement (FEM_model_bsr.py:134-145, 476-598) ----------------
+
+    def LDinit(self, displace):
+        """FEM_model_bsr.py:588-598: the displacement (3N,) the large-displacement
+        matrices are formed at (kept on the device)."""
+        if torch.is_tensor(displace):
+            d = displace.detach().to(device=device(), dtype=torch.float64).reshape(-1)
+        else:
+            d = torch.as_tensor(np.asarray(displace, dtype=np.float64).reshape(-1)).to(device())
+        if d.numel() != 3 * self.Num_Node:
+            raise ValueError(f"displacement must have {3 * self.Num_Node} entries")
+        self.U_ld = d.contiguous().clone()
+
+    def assemble_KN(self, layout="bsr", penalty=True):
+        """(KT, KM) values (9*nnzb each) on the device at the displacement of
+        LDinit (zero if never set): the tangent and the matrix of the residual
+        KM U - F (UnitK, FEM_model_bsr.py:476-505), penalty on both (:248-259)."""
+        if getattr(self, "U_ld", None) is None:
+            self.U_ld = torch.zeros(3 * self.Num_Node, dtype=torch.float64, device=device())
+        kdev = self.Physics.kn_dev
+        lay = _lib.LAYOUT_BSR if layout == "bsr" else _lib.LAYOUT_CSR
+        kt = torch.empty(9 * self.nnzb, dtype=torch.float64, device=device())
+        km = torch.empty_like(kt)
+        _lib.call("fem_hex8_assemble_kn", kdev.ref, self._young_ptr(), self.young_scalar, float(self.PO),
+                  self.U_ld.data_ptr(), kt.data_ptr(), km.data_ptr(), lay, stream_ptr())
+        if penalty:
+            self.apply_penalty(kt, "f64", layout)
+            self.apply_penalty(km, "f64", layout)
+        return kt, km
+
+    def Get_MatrixKN(self):
+        """FEM_model_bsr.py:134-145: (KT, KM) as scipy CSR (3N x 3N)."""
+        kt, km = self.assemble_KN("csr")
+        row, col = self._csr_index_K()
+        n = self.Num_Free
+        return tuple(sp.csr_matrix((v.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64) for v in (kt, km))
+
+    def _csr_index_K(self):
+        """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173), host copies."""
         d = device()
         row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
         col = torch.empty(max(self.nnz_K, 1), dtype=torch.int32, device=d)
         _lib.call("fem_csr_index_K", self.dev.ref, row.data_ptr(), col.data_ptr(), stream_ptr())
+        return row.cpu().numpy(), col[:self.nnz_K].cpu().numpy()
+
+    def Get_MatrixKL(self):
+        """FEM_model_csr.py:118-127: penalised stiffness as scipy CSR (3N x 3N)."""
+        vals = self.assemble_K("f64", "csr")
+        row, col = self._csr_index_K()
         n = self.Num_Free
-        return sp.csr_matrix((vals.cpu().numpy(), col[:self.nnz_K].cpu().numpy(), row.cpu().numpy()),
-                             shape=(n, n), dtype=np.float64)
+        return sp.csr_matrix((vals.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64)
 
     Get_MatrixK = Get_MatrixKL
 

@@ -22,6 +22,12 @@ from .runtime import device, stream_ptr
 
 # Upper bound on the elements one row group may touch (LDS staging budget).
 MAX_GROUP_ELEMS = 48
+# Large-displacement assembly: ~3 KB of LDS per element (gradients, F, F F^T,
+# stress), so smaller groups: 2 x 2 x 1 node bricks (18 elements), or 4 rows
+# of the Morton order, at most 24 elements.
+KN_BRICK = (2, 2, 1)
+KN_MORTON_ROWS = 4
+KN_MAX_GROUP_ELEMS = 24
 
 
 class DeviceMesh:
@@ -144,4 +150,27 @@ class Physics:
         if lds > 160 * 1024:
             raise ValueError(f"row groups need {lds} B of LDS; use a smaller brick")
         self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
+        self._symbolic = (coords, pat, ne_ptr, ne_list)
+        self._kn_dev = None
         return self
+
+    @property
+    def kn_dev(self):
+        """The same mesh and pattern with the smaller row groups of the
+        large-displacement assembly (built on first use)."""
+        if self._kn_dev is None:
+            coords, pat, ne_ptr, ne_list = self._symbolic
+            gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=KN_BRICK,
+                                 rows=KN_MORTON_ROWS)
+            groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, KN_MAX_GROUP_ELEMS,
+                                      co_node=self.CoNode)
+            tp, tasks = build_tasks(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr, ne_list,
+                                    groups)
+            groups["grp_task_ptr"], groups["tasks"] = tp, tasks
+            groups["grp_enodes"] = np.ascontiguousarray(self.elem_nodes[groups["grp_elems"]].reshape(-1))
+            groups["max_group_tasks"] = int(np.diff(tp).max()) if len(tp) > 1 else 0
+            lds = _lib.lib().fem_hex8_kn_lds_bytes(groups["max_group_elems"], groups["max_group_rows"])
+            if lds > 160 * 1024:
+                raise ValueError(f"large-displacement row groups need {lds} B of LDS")
+            self._kn_dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
+        return self._kn_dev

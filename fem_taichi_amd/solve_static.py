@@ -7,10 +7,20 @@ by one of the GPU forms of solvers.py, chosen by the extra keyword
 refinement, 'learned' flexible PCG with the mesh-convolution
 preconditioner).  ``tol`` is the FP64 relative residual target; the achieved
 value is printed as the reference prints its Pardiso residual and kept in
-``self.info``.  Large-displacement Newton (Qtype=1) is not part of this
-package.
+``self.info``.
+
+``Qtype=1`` is the reference's large-displacement Newton loop
+(SOLVE_static.py:41-76): at each step LDinit(U), (KT, KM) = Get_MatrixKN(),
+residual RF = KM U - F, stop when ||RF|| / ||F|| <= max_errs or after
+max_iters steps.  The update solves KT dU = RF (FP64 block-Jacobi PCG on the
+GPU) and sets U = U - dU.  The reference passes F instead of RF to its
+solve (SOLVE_static.py:70), which from U = 0 steps to U = -K^-1 F and does
+not converge; the residual form is used here and ``self.history`` keeps the
+residual of every step.
 """
 from __future__ import annotations
+
+import torch
 
 from . import solvers
 
@@ -22,8 +32,10 @@ class Static_solver:
         self.info = None
 
     def Solve_cpu(self, max_errs=1e-4, max_iters=20, method="pcg", tol=1e-8, verbose=False, **kw):
+        if self.Qtype == 1:
+            return self._solve_newton(max_errs, max_iters, tol, verbose)
         if self.Qtype != 0:
-            raise NotImplementedError("only the linear static case (Qtype=0) is implemented")
+            raise ValueError(f"unknown Qtype {self.Qtype!r}")
         M = self.Model
         F = M.F_device()
         K64 = solvers.BsrOperator(M.dev, M.K_bsr("f64"))
@@ -42,4 +54,30 @@ class Static_solver:
         if verbose:
             print(f"\t\t\t{info.method}: {info.iterations} iterations, relative residual "
                   f"{info.rel_residual:.3e}, {info.seconds:.3f} s")
+        return U.cpu().numpy()
+
+    def _solve_newton(self, max_errs, max_iters, tol, verbose):
+        M = self.Model
+        F = M.F_device()
+        nf = float(torch.linalg.vector_norm(F))
+        U = torch.zeros_like(F)
+        self.history = []
+        step = 0
+        while True:
+            M.LDinit(U)
+            kt, km = M.assemble_KN("bsr")
+            KT = solvers.BsrOperator(M.dev, kt)
+            KM = solvers.BsrOperator(M.dev, km)
+            RF = KM.matvec(U) - F
+            err = float(torch.linalg.vector_norm(RF)) / nf if nf > 0 else 0.0
+            self.history.append(err)
+            if verbose:
+                print(f"\t\t\tNewton step {step + 1}: relative residual {err:.3e}")
+            if err <= max_errs or step == max_iters:
+                break
+            dU, info = solvers.solve_pcg64(KT, RF, tol=tol)
+            U = U - dU
+            step += 1
+        self.U_device = U
+        self.info = {"steps": step, "residual": self.history[-1]}
         return U.cpu().numpy()

@@ -142,6 +142,18 @@ int fem_hex8_assemble(const fem_mesh* mesh, const double* young, double young_sc
 int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_rows, int k_dtype,
                                     int want_k, int want_m);
 
+/* ---- large displacement (replaces UnitK / InstallMatrixKN / LDinit,
+ *      FEM_model_bsr.py:178-191, 436-449, 476-598, behind Get_MatrixKN :134-145)
+ * At the displacement u (3N, device), writes the tangent KT and the matrix KM
+ * of the residual KM u - F (KM is not symmetric) into kt_vals / km_vals (9*nnzb
+ * f64 each, k_layout).  The geometric term uses the element stress summed
+ * over the Gauss points visited so far, as the reference's element field S
+ * is.  Needs row groups of at most 32 elements (the fem_mesh given is one
+ * built for this kernel's LDS budget: fem_hex8_kn_lds_bytes). */
+int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
+                         const double* u, double* kt_vals, double* km_vals, int k_layout, void* stream);
+int64_t fem_hex8_kn_lds_bytes(int32_t max_group_elems, int32_t max_group_rows);
+
 /* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and
  *      ModifyMatrixK FEM_model_bsr.py:248-259) ------------------------------
  * For each bound entry b (node index bound_node[b], resolved by the host

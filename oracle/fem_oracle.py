@@ -26,12 +26,17 @@ get_stress             POST_process.py:72-99, 172-223, 277-321     uniform-strai
                                                                    source text only
 lanczos                SOLVE_eigen.py:45-89                        dense scipy eigh on small pencils,
                                                                    the SOLVE_eigen.py:101-117 pencil
+element_KN             FEM_model_bsr.py:476-598 (UnitK, BN, S)     U = 0 gives element_K; rigid
+                                                                   translation of U invariance;
+                                                                   independent F-based closed form
 =====================  ==========================================  ==============================
 
 Known reference quirks restated on purpose (see the module docstrings of each
 function): Q1 penalty *sets* 1e20; Q3 the Get_Stress extrapolation evaluates
 ``shapefunc_GS`` at RMat-ordered corners with shape-ordered Gauss weights;
-Q4 the von Mises term ``(p2-p1)^2``; Q6 zero-padded T in block Lanczos.
+Q4 the von Mises term ``(p2-p1)^2``; Q6 zero-padded T in block Lanczos; QN
+the geometric stiffness of UnitK uses the element stress accumulated over the
+Gauss points visited so far (S is an element field summed in the Gauss loop).
 
 Parity of the extrapolation map of Q3 and of the von Mises quirk Q4 cannot be
 pinned by a closed form (a patch test is blind to them: constant fields pass
@@ -189,6 +194,93 @@ def element_K(Coord, EX, PO):
             t2 = np.einsum('ejrp,rs,ekst->ejkpt', B2, D1, B2, optimize=True)
         K += det[:, None, None, None, None] * (t1 + t2)
     return K
+
+
+def _local_B_bsr(DN):
+    """FEM_model_bsr.py:303-347 (installLocalB): B1 (normal) and B2 with the
+    shear rows in the BSR model's order yz, xz, xy.  (..., 8, 3, 3)."""
+    sh = DN.shape[:-1] + (3, 3)
+    B1 = np.zeros(sh)
+    B2 = np.zeros(sh)
+    for d in range(3):
+        B1[..., d, d] = DN[..., d]
+    B2[..., 0, 1] = DN[..., 2]; B2[..., 0, 2] = DN[..., 1]
+    B2[..., 1, 0] = DN[..., 2]; B2[..., 1, 2] = DN[..., 0]
+    B2[..., 2, 0] = DN[..., 1]; B2[..., 2, 1] = DN[..., 0]
+    return B1, B2
+
+
+def _local_BN(T, DN):
+    """FEM_model_bsr.py:548-585 (InslocalBN): BN1[j][p, k] = T[k, p] DN[j, p];
+    BN2 rows yz, xz, xy: T[k,2] DN[j,1] + T[k,1] DN[j,2], T[k,2] DN[j,0] +
+    T[k,0] DN[j,2], T[k,1] DN[j,0] + T[k,0] DN[j,1].  T (E, 3, 3), DN (E, 8, 3)."""
+    Tt = np.transpose(T, (0, 2, 1))[:, None]                        # Tt[e,0,p,k] = T[e,k,p]
+    BN1 = DN[..., :, None] * Tt                                     # (E,8,3,3)
+    BN2 = np.empty_like(BN1)
+    BN2[..., 0, :] = T[:, None, :, 2] * DN[..., 1, None] + T[:, None, :, 1] * DN[..., 2, None]
+    BN2[..., 1, :] = T[:, None, :, 2] * DN[..., 0, None] + T[:, None, :, 0] * DN[..., 2, None]
+    BN2[..., 2, :] = T[:, None, :, 1] * DN[..., 0, None] + T[:, None, :, 0] * DN[..., 1, None]
+    return BN1, BN2
+
+
+def element_KN(Coord, CoordU, EX, PO):
+    """FEM_model_bsr.py:476-505 (UnitK) after LDinit/initK (:588-598, :436-449):
+    the large-displacement tangent KT and the matrix KM of the residual
+    KM U - F, both (E, 8, 8, 3, 3).
+
+    For each Gauss point g (RMat order): T = CoordU @ DN (displacement
+    gradient), L_j = B_j + BN_j, for every node j the element stress
+    accumulates S += D (B_j + BN_j / 2) u_j, then for every (j, k)
+        KT[j,k] += detJ (L_j^T D L_k),  KM[j,k] += detJ (L_j^T D (B_k + BN_k / 2)),
+    and, with s the 3x3 tensor of the stress accumulated so far (QN),
+        KT[j,k] += detJ (DN_j^T s DN_k) I.
+    ``EX`` may be a per-element array (extension; the reference has one EX)."""
+    Coord = np.asarray(Coord, dtype=np.float64)
+    CoordU = np.asarray(CoordU, dtype=np.float64)
+    E = Coord.shape[0]
+    D0, D1 = dmat(EX, PO)
+    if D0.ndim == 2:
+        D0 = np.broadcast_to(D0, (E, 3, 3))
+        D1 = np.broadcast_to(D1, (E, 3, 3))
+    KT = np.zeros((E, 8, 8, 3, 3))
+    KM = np.zeros((E, 8, 8, 3, 3))
+    S0 = np.zeros((E, 3))
+    S1 = np.zeros((E, 3))                                            # yz, xz, xy
+    u = np.transpose(CoordU, (0, 2, 1))                              # (E, 8, 3)
+    for g in range(8):
+        DNr = shape_derv(RMAT[g])
+        J = Coord @ DNr
+        det = np.linalg.det(J)
+        DN = DNr[None] @ np.linalg.inv(J)                            # (E, 8, 3)
+        T = CoordU @ DN                                              # (E, 3, 3)
+        B1, B2 = _local_B_bsr(DN)
+        BN1, BN2 = _local_BN(T, DN)
+        H1, H2 = B1 + 0.5 * BN1, B2 + 0.5 * BN2
+        S0 += np.einsum('eps,es->ep', D0, np.einsum('ejsk,ejk->es', H1, u))
+        S1 += np.einsum('eps,es->ep', D1, np.einsum('ejsk,ejk->es', H2, u))
+        L1, L2 = B1 + BN1, B2 + BN2
+        lt1 = np.einsum('ejrp,ers->ejps', L1, D0)
+        lt2 = np.einsum('ejrp,ers->ejps', L2, D1)
+        AB = np.einsum('ejps,eksq->ejkpq', lt1, L1) + np.einsum('ejps,eksq->ejkpq', lt2, L2)
+        AhB = np.einsum('ejps,eksq->ejkpq', lt1, H1) + np.einsum('ejps,eksq->ejkpq', lt2, H2)
+        w = det[:, None, None, None, None]
+        KT += w * AB
+        KM += w * AhB
+        s = np.zeros((E, 3, 3))                                      # MatrixS (:469-475)
+        s[:, 0, 0], s[:, 1, 1], s[:, 2, 2] = S0[:, 0], S0[:, 1], S0[:, 2]
+        s[:, 0, 1] = s[:, 1, 0] = S1[:, 2]
+        s[:, 0, 2] = s[:, 2, 0] = S1[:, 1]
+        s[:, 1, 2] = s[:, 2, 1] = S1[:, 0]
+        geo = np.einsum('ejp,epq,ekq->ejk', DN, s, DN) * det[:, None, None]
+        KT += geo[..., None, None] * np.eye(3)
+    return KT, KM
+
+
+def element_coord_u(U, Elem, Nind):
+    """FEM_model_bsr.py:443-446: CoordU[i][k, j] = U[Nind[Elem[j, i]]][k] -> (E, 3, 8)."""
+    en = element_nodes(Elem, Nind)
+    u = np.asarray(U, dtype=np.float64).reshape(-1, 3)
+    return np.transpose(u[en], (0, 2, 1)).copy()
 
 
 def element_M(Coord, DE):
@@ -413,6 +505,19 @@ def model_K_bsr_values(Node, Elem, pp, EX, PO, Bounds):
     Ke = element_K(element_coords(Node, Elem, pp['Nind']), EX, PO)
     V = assemble_K_bsr(Ke, en, pp['SmNode'], pp['ReNode'])
     return modify_K_bsr(V, stage_bounds(Bounds), pp['Nind'], pp['SmNode'], pp['ReNode'])
+
+
+def model_KN_bsr_values(Node, Elem, pp, EX, PO, Bounds, U):
+    """Model.LDinit(U); Model.Get_MatrixKN() (FEM_model_bsr.py:134-145):
+    (KT, KM) BSR values (nnzb, 3, 3), the penalty set on both (:248-259)."""
+    en = element_nodes(Elem, pp['Nind'])
+    KTe, KMe = element_KN(element_coords(Node, Elem, pp['Nind']), element_coord_u(U, Elem, pp['Nind']), EX, PO)
+    b = stage_bounds(Bounds)
+    out = []
+    for Ke in (KTe, KMe):
+        V = assemble_K_bsr(Ke, en, pp['SmNode'], pp['ReNode'])
+        out.append(modify_K_bsr(V, b, pp['Nind'], pp['SmNode'], pp['ReNode']))
+    return out[0], out[1]
 
 
 def model_M_csr(Node, Elem, pp, DE):
