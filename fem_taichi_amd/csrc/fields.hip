@@ -10,58 +10,77 @@
 namespace fem {
 namespace {
 
-// InstallVectorF (FEM_model_csr.py:214-229) with UnitVectorF (:275-283):
-// Fe[k] = sum_g rho*detJ_g * N_gk * (|w|^2 R_g - (w.R_g) w), R_g = sum_k N_gk x_k.
-__global__ void body_force_elem(fem_mesh m, double rho, double w0, double w1, double w2,
-                                double* __restrict__ fe) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m.num_elem) return;
-  double x[8][3];
-  int32_t nd[8];
+// Element kernels: one lane per (element, Gauss point), 32 elements per
+// work-group.  The lane computes its Gauss point's quantities into LDS; after
+// a barrier lane (element, k) reduces over the 8 Gauss points for local node
+// k (in Gauss order), so no lane holds an element's 8 x 3 (or 8 x 6) outputs.
+constexpr int kFT = 256;
+constexpr int kFE = kFT / 8;
+
+// J at Gauss point g of the element with nodes nd, J[a][d] = dx_a/dxi_d
+// (the x_0 term of sum_k x_k (x) dN_k vanishes: sum_k dN_k = 0).
+__device__ __forceinline__ void jacobian_at(const fem_mesh& m, const int32_t nd[8], int g, double J[9], double R[3]) {
+  const int64_t n0 = nd[0];
+  const double x00 = m.coords[n0 * 3], x01 = m.coords[n0 * 3 + 1], x02 = m.coords[n0 * 3 + 2];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) J[q] = 0.0;
+  R[0] = R[1] = R[2] = 0.0;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
-    nd[k] = m.elem[e * 8 + k];
+    const int64_t nk = nd[k];
+    const double x[3] = {m.coords[nk * 3], m.coords[nk * 3 + 1], m.coords[nk * 3 + 2]};
+    const double Nk = nshape(g, k);
+    const double dn[3] = {dnr(g, k, 0), dnr(g, k, 1), dnr(g, k, 2)};
+    const double x0[3] = {x00, x01, x02};
 #pragma unroll
-    for (int a = 0; a < 3; ++a) x[k][a] = m.coords[(int64_t)nd[k] * 3 + a];
-  }
-  double acc[8][3];
+    for (int a = 0; a < 3; ++a) {
+      R[a] = fma(Nk, x[a], R[a]);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
-#pragma unroll
-  for (int g = 0; g < 8; ++g) {
-    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, R[3] = {0, 0, 0}, N[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double f[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) f[a] = 1.0 + node_sign(k, a) * gauss_sign(g, a) * kGS;
-      N[k] = 0.125 * f[0] * f[1] * f[2];
-      double dn[3] = {0.125 * node_sign(k, 0) * f[1] * f[2], 0.125 * node_sign(k, 1) * f[0] * f[2],
-                      0.125 * node_sign(k, 2) * f[0] * f[1]};
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        R[a] = fma(N[k], x[k][a], R[a]);
-        const double xr = x[k][a] - x[0][a];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr, dn[d], J[a * 3 + d]);
-      }
-    }
-    const double F0 = w1 * w1 * R[0] + w2 * w2 * R[0] - w0 * w1 * R[1] - w0 * w2 * R[2];
-    const double F1 = w0 * w0 * R[1] + w2 * w2 * R[1] - w1 * w2 * R[2] - w1 * w0 * R[0];
-    const double F2 = w1 * w1 * R[2] + w0 * w0 * R[2] - w2 * w0 * R[0] - w2 * w1 * R[1];
-    const double jd = rho * det3(J);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const double c = jd * N[k];
-      acc[k][0] = fma(c, F0, acc[k][0]);
-      acc[k][1] = fma(c, F1, acc[k][1]);
-      acc[k][2] = fma(c, F2, acc[k][2]);
+      for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(x[a] - x0[a], dn[d], J[a * 3 + d]);
     }
   }
+}
+
+__device__ __forceinline__ void element_nodes8(const fem_mesh& m, int64_t e, int32_t nd[8]) {
+  const int4* en = reinterpret_cast<const int4*>(m.elem + e * 8);
+  const int4 a = en[0], b = en[1];
+  nd[0] = a.x; nd[1] = a.y; nd[2] = a.z; nd[3] = a.w;
+  nd[4] = b.x; nd[5] = b.y; nd[6] = b.z; nd[7] = b.w;
+}
+
+// InstallVectorF (FEM_model_csr.py:214-229) with UnitVectorF (:275-283):
+// Fe[k] = sum_g rho*detJ_g * N_gk * (|w|^2 R_g - (w.R_g) w), R_g = sum_k N_gk x_k.
+__global__ void __launch_bounds__(kFT)
+body_force_elem(fem_mesh m, double rho, double w0, double w1, double w2, double* __restrict__ fe) {
+  __shared__ double s_d[kFE][8][4];   // rho*detJ, F(R) per (element, Gauss point)
+  const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
+  const int64_t e = (int64_t)blockIdx.x * kFE + le;
+  const bool valid = e < m.num_elem;
+  if (valid) {
+    int32_t nd[8];
+    element_nodes8(m, e, nd);
+    double J[9], R[3];
+    jacobian_at(m, nd, g, J, R);
+    s_d[le][g][0] = rho * det3(J);
+    s_d[le][g][1] = w1 * w1 * R[0] + w2 * w2 * R[0] - w0 * w1 * R[1] - w0 * w2 * R[2];
+    s_d[le][g][2] = w0 * w0 * R[1] + w2 * w2 * R[1] - w1 * w2 * R[2] - w1 * w0 * R[0];
+    s_d[le][g][3] = w1 * w1 * R[2] + w0 * w0 * R[2] - w2 * w0 * R[0] - w2 * w1 * R[1];
+  }
+  __syncthreads();
+  if (valid) {
+    const int k = g;   // this lane now owns local node k
+    double a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-#pragma unroll
-    for (int a = 0; a < 3; ++a) fe[e * 24 + k * 3 + a] = acc[k][a];
+    for (int q = 0; q < 8; ++q) {
+      const double c = s_d[le][q][0] * nshape(q, k);
+      a0 = fma(c, s_d[le][q][1], a0);
+      a1 = fma(c, s_d[le][q][2], a1);
+      a2 = fma(c, s_d[le][q][3], a2);
+    }
+    fe[e * 24 + k * 3] = a0;
+    fe[e * 24 + k * 3 + 1] = a1;
+    fe[e * 24 + k * 3 + 2] = a2;
+  }
 }
 
 // LoadF (FEM_model_csr.py:205-210) after the body force: per-node gather.
@@ -89,80 +108,62 @@ __global__ void body_force_gather(fem_mesh m, const double* __restrict__ fe,
 // UnitVectorS (POST_process.py:190-209) and the extrapolation of get_NdSol
 // (:172-181): per element, Gauss-point stresses, then per local node j
 // sum_g shapefunc_GS(GMat[j])[g] * SVec[g] -> se[e][j][6].
-__global__ void stress_elem(fem_mesh m, const double* __restrict__ young, double young_scalar,
-                            double lhat, double mhat, const double* __restrict__ u,
-                            double* __restrict__ se) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m.num_elem) return;
-  double x[8][3], U[8][3];
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int64_t nk = m.elem[e * 8 + k];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      x[k][a] = m.coords[nk * 3 + a];
-      U[k][a] = u[nk * 3 + a];
-    }
-  }
-  const double Ee = young ? young[e] : young_scalar;
-  const double lam = Ee * lhat, mu = Ee * mhat;
-  double out[8][6];
-#pragma unroll
-  for (int j = 0; j < 8; ++j)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) out[j][c] = 0.0;
-#pragma unroll
-  for (int g = 0; g < 8; ++g) {
-    double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    double dn[8][3];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      double f[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) f[a] = 1.0 + node_sign(k, a) * gauss_sign(g, a) * kGS;
-      dn[k][0] = 0.125 * node_sign(k, 0) * f[1] * f[2];
-      dn[k][1] = 0.125 * node_sign(k, 1) * f[0] * f[2];
-      dn[k][2] = 0.125 * node_sign(k, 2) * f[0] * f[1];
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const double xr = x[k][a] - x[0][a];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(xr, dn[k][d], J[a * 3 + d]);
-      }
-    }
-    double Ji[9];
+__global__ void __launch_bounds__(kFT)
+stress_elem(fem_mesh m, const double* __restrict__ young, double young_scalar, double lhat, double mhat,
+            const double* __restrict__ u, double* __restrict__ se) {
+  __shared__ double s_s[kFE][8][6];   // Gauss-point stress (xx yy zz xy yz xz)
+  const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
+  const int64_t e = (int64_t)blockIdx.x * kFE + le;
+  const bool valid = e < m.num_elem;
+  if (valid) {
+    int32_t nd[8];
+    element_nodes8(m, e, nd);
+    double J[9], R[3], Ji[9];
+    jacobian_at(m, nd, g, J, R);
     inv3(J, Ji);
+    const double Ee = young ? young[e] : young_scalar;
+    const double lam = Ee * lhat, mu = Ee * mhat;
     double exx = 0, eyy = 0, ezz = 0, gxy = 0, gyz = 0, gxz = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
+      const double dn[3] = {dnr(g, k, 0), dnr(g, k, 1), dnr(g, k, 2)};
       double D[3];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) D[a] = dn[k][0] * Ji[a] + dn[k][1] * Ji[3 + a] + dn[k][2] * Ji[6 + a];
-      exx = fma(D[0], U[k][0], exx);
-      eyy = fma(D[1], U[k][1], eyy);
-      ezz = fma(D[2], U[k][2], ezz);
-      gxy += D[1] * U[k][0] + D[0] * U[k][1];
-      gyz += D[2] * U[k][1] + D[1] * U[k][2];
-      gxz += D[2] * U[k][0] + D[0] * U[k][2];
+      for (int a = 0; a < 3; ++a) D[a] = dn[0] * Ji[a] + dn[1] * Ji[3 + a] + dn[2] * Ji[6 + a];
+      const int64_t nk = nd[k];
+      const double U0 = u[nk * 3], U1 = u[nk * 3 + 1], U2 = u[nk * 3 + 2];
+      exx = fma(D[0], U0, exx);
+      eyy = fma(D[1], U1, eyy);
+      ezz = fma(D[2], U2, ezz);
+      gxy += D[1] * U0 + D[0] * U1;
+      gyz += D[2] * U1 + D[1] * U2;
+      gxz += D[2] * U0 + D[0] * U2;
     }
     const double tr = lam * (exx + eyy + ezz);
-    const double sv[6] = {tr + 2 * mu * exx, tr + 2 * mu * eyy, tr + 2 * mu * ezz,
-                          mu * gxy, mu * gyz, mu * gxz};
+    s_s[le][g][0] = tr + 2 * mu * exx;
+    s_s[le][g][1] = tr + 2 * mu * eyy;
+    s_s[le][g][2] = tr + 2 * mu * ezz;
+    s_s[le][g][3] = mu * gxy;
+    s_s[le][g][4] = mu * gyz;
+    s_s[le][g][5] = mu * gxz;
+  }
+  __syncthreads();
+  if (valid) {
+    const int j = g;   // this lane now owns local node j
+    double out[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      // W[j][g] = shapefunc_GS(GMat[j])[g]: GMat in Gauss-point sign order,
-      // weight index g taken with the shape-node sign table (as written).
+    for (int q = 0; q < 8; ++q) {
+      // W[j][q] = shapefunc_GS(GMat[j])[q]: GMat in Gauss-point sign order,
+      // weight index q taken with the shape-node sign table (as written).
       double w = 0.125;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) w *= 1.0 + node_sign(g, a) * gauss_sign(j, a) / kGS;
+      for (int a = 0; a < 3; ++a) w *= 1.0 + node_sign(q, a) * gauss_sign(j, a) / kGS;
 #pragma unroll
-      for (int c = 0; c < 6; ++c) out[j][c] = fma(w, sv[c], out[j][c]);
+      for (int c = 0; c < 6; ++c) out[c] = fma(w, s_s[le][q][c], out[c]);
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) se[e * 48 + j * 6 + c] = out[c];
   }
-#pragma unroll
-  for (int j = 0; j < 8; ++j)
-#pragma unroll
-    for (int c = 0; c < 6; ++c) se[e * 48 + j * 6 + c] = out[j][c];
 }
 
 // Symmetric 3x3 eigenvalues by cyclic Jacobi rotations (FP64), descending.
@@ -251,10 +252,11 @@ extern "C" int fem_body_force(const fem_mesh* mesh, double rho, const double* om
   FEM_REQUIRE(omega3 != nullptr, FEM_E_INVALID, "omega3 is NULL (host pointer to 3 doubles)");
   FEM_REQUIRE(mesh->num_node == 0 || f_out, FEM_E_INVALID, "f_out is NULL");
   FEM_REQUIRE(work_bytes >= fem_body_force_work_bytes(mesh->num_elem), FEM_E_INVALID, "work too small");
+  FEM_REQUIRE(((uintptr_t)mesh->elem & 15) == 0, FEM_E_INVALID, "elem must be 16-byte aligned");
   hipStream_t st = as_stream(stream);
   double* fe = reinterpret_cast<double*>(work);
   if (mesh->num_elem > 0) {
-    body_force_elem<<<ceil_div(mesh->num_elem, 128), 128, 0, st>>>(*mesh, rho, omega3[0], omega3[1],
+    body_force_elem<<<ceil_div(mesh->num_elem, kFE), kFT, 0, st>>>(*mesh, rho, omega3[0], omega3[1],
                                                                     omega3[2], fe);
     FEM_LAUNCH_CHECK();
   }
@@ -272,13 +274,14 @@ extern "C" int fem_stress_recover(const fem_mesh* mesh, const double* young, dou
   if (rc) return rc;
   FEM_REQUIRE(mesh->num_node == 0 || (u && out), FEM_E_INVALID, "u/out NULL");
   FEM_REQUIRE(work_bytes >= fem_stress_work_bytes(mesh->num_elem), FEM_E_INVALID, "work too small");
+  FEM_REQUIRE(((uintptr_t)mesh->elem & 15) == 0, FEM_E_INVALID, "elem must be 16-byte aligned");
   FEM_REQUIRE(nu > -1.0 && nu < 0.5, FEM_E_INVALID, "Poisson ratio %g outside (-1, 0.5)", nu);
   hipStream_t st = as_stream(stream);
   double* se = reinterpret_cast<double*>(work);
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
   if (mesh->num_elem > 0) {
-    stress_elem<<<ceil_div(mesh->num_elem, 64), 64, 0, st>>>(*mesh, young, young_scalar, lhat, mhat,
+    stress_elem<<<ceil_div(mesh->num_elem, kFE), kFT, 0, st>>>(*mesh, young, young_scalar, lhat, mhat,
                                                               u, se);
     FEM_LAUNCH_CHECK();
   }

@@ -72,7 +72,7 @@ typedef struct fem_mesh {
   int64_t num_elem;
   int64_t nnzb;
   const double* coords;        /* N x 3 */
-  const int32_t* elem;         /* E x 8 */
+  const int32_t* elem;         /* E x 8, 16-byte aligned */
   const int32_t* sm_node;      /* N + 1 */
   const int32_t* re_node;      /* nnzb  */
   const int32_t* co_node;      /* N     */
