@@ -54,11 +54,19 @@ class FemMesh(ctypes.Structure):
     ]
 
 
+class GridTransfer(ctypes.Structure):
+    """Mirror of ``struct fem_grid_transfer`` (include/fem355.h)."""
+
+    _fields_ = [("nf", ctypes.c_int32 * 3), ("nc", ctypes.c_int32 * 3), ("left", ctypes.c_void_p),
+                ("wright", ctypes.c_void_p), ("span", ctypes.c_void_p)]
+
+
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _I32 = ctypes.c_int
 _D = ctypes.c_double
 _MP = ctypes.POINTER(FemMesh)
+_TP = ctypes.POINTER(GridTransfer)
 
 # name -> (restype, argtypes); every name here is declared in include/fem355.h
 SIGNATURES = {
@@ -99,6 +107,10 @@ SIGNATURES = {
     "fem_relu_bwd_bf16": (_I32, [_P, _P, _I64, _P, _P]),
     "fem_to_bf16": (_I32, [_P, _I64, _I32, _I32, _P, _P]),
     "fem_pcg_replace_residual": (_I32, [_MP, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+    "fem_mg_smooth": (_I32, [_MP, _P, _P, ctypes.c_float, _P, _P, _P, _I32, _P]),
+    "fem_mg_restrict": (_I32, [_TP, _P, _P, _P]),
+    "fem_mg_prolong_add": (_I32, [_TP, _P, _P, _P]),
+    "fem_dense_gemv_f32": (_I32, [_P, _I64, _P, _P, _P]),
 }
 
 _lib = None

@@ -1,0 +1,245 @@
+// Geometric multigrid on structured node grids: the FP32 V-cycle that
+// preconditions the FP64 flexible CG of the static solve (the GPU solver
+// that replaces the Pardiso call of SOLVE_static.py:31-38).
+//
+// * Grid transfers between a node grid (x fastest) and its coarsening, which
+//   keeps every second node along each axis plus the last one.  Prolongation
+//   is trilinear in the index coordinate, restriction its transpose.  Both
+//   are gathers (thread per output node), so no atomics and a fixed
+//   summation order.
+// * The damped block-Jacobi smoother x <- x + w Dinv (b - K x) and the
+//   residual r = b - K x on the BSR(3x3) FP32 matrix: 32 lanes per block row
+//   (lane = block of the row, so the row's contiguous run of blocks is read
+//   coalesced), the three row sums reduced with lane shuffles.
+// * y = A x for the small dense inverse of the coarsest level.
+#include "common.hpp"
+
+namespace fem {
+namespace {
+
+constexpr int kLanes = 32;
+constexpr int kThreads = 256;
+constexpr int kRowsPerBlock = kThreads / kLanes;
+
+// (K x)_i for block row i over the row's blocks; lanes b, b + 32, ...
+__device__ __forceinline__ void row_mul(const fem_mesh& m, const float* __restrict__ vals,
+                                        const float* __restrict__ x, int64_t i, int lane, float& a0,
+                                        float& a1, float& a2) {
+  a0 = a1 = a2 = 0.0f;
+  const int64_t sm = m.sm_node[i];
+  const int co = m.co_node[i];
+  for (int b = lane; b < co; b += kLanes) {
+    const float* __restrict__ v = vals + 9 * (sm + b);
+    const int64_t c = 3 * (int64_t)m.re_node[sm + b];
+    const float x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+    a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+    a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+    a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+  }
+#pragma unroll
+  for (int o = kLanes / 2; o > 0; o >>= 1) {
+    a0 += __shfl_xor(a0, o, 64);
+    a1 += __shfl_xor(a1, o, 64);
+    a2 += __shfl_xor(a2, o, 64);
+  }
+}
+
+// MODE 0: out = b - K x.  MODE 1: out = x + w Dinv (b - K x) (out != x).
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+smooth_kernel(fem_mesh m, const float* __restrict__ vals, const float* __restrict__ dinv, float w,
+              const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ out) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanes;
+  // the 32 lanes of a row take the same branch, so the shuffles see whole groups
+  if (i < m.num_node) {
+    float a0, a1, a2;
+    row_mul(m, vals, x, i, lane, a0, a1, a2);
+    if (lane == 0) {
+      const float r0 = b[3 * i] - a0, r1 = b[3 * i + 1] - a1, r2 = b[3 * i + 2] - a2;
+      if (MODE == 0) {
+        out[3 * i] = r0;
+        out[3 * i + 1] = r1;
+        out[3 * i + 2] = r2;
+      } else {
+        const float* __restrict__ D = dinv + 9 * i;
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+          out[3 * i + p] = x[3 * i + p] + w * (D[3 * p] * r0 + D[3 * p + 1] * r1 + D[3 * p + 2] * r2);
+      }
+    }
+  }
+}
+
+// out = w Dinv b (the smoother's first step from x = 0).
+__global__ void __launch_bounds__(kThreads)
+jacobi_start_kernel(int64_t N, const float* __restrict__ dinv, float w, const float* __restrict__ b,
+                    float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= N) return;
+  const float* __restrict__ D = dinv + 9 * i;
+  const float b0 = b[3 * i], b1 = b[3 * i + 1], b2 = b[3 * i + 2];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) out[3 * i + p] = w * (D[3 * p] * b0 + D[3 * p + 1] * b1 + D[3 * p + 2] * b2);
+}
+
+// Weight of coarse index c for fine index f along one axis.
+__device__ __forceinline__ float axis_weight(const int32_t* __restrict__ left, const float* __restrict__ wr,
+                                             int f, int c) {
+  const int l = left[f];
+  return l == c ? 1.0f - wr[f] : (l + 1 == c ? wr[f] : 0.0f);
+}
+
+// coarse[I] = sum over fine nodes f of P[f, I] fine[f] (thread per coarse node).
+__global__ void __launch_bounds__(kThreads)
+restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __restrict__ coarse) {
+  const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t ncx = t.nc[0], ncy = t.nc[1];
+  if (I >= ncx * ncy * t.nc[2]) return;
+  const int cx = (int)(I % ncx), cy = (int)((I / ncx) % ncy), cz = (int)(I / (ncx * ncy));
+  const int32_t* lx = t.left;
+  const int32_t* ly = lx + t.nf[0];
+  const int32_t* lz = ly + t.nf[1];
+  const float* wx = t.wright;
+  const float* wy = wx + t.nf[0];
+  const float* wz = wy + t.nf[1];
+  const int32_t* sx = t.span;
+  const int32_t* sy = sx + 2 * t.nc[0];
+  const int32_t* sz = sy + 2 * t.nc[1];
+  const int64_t fx0 = t.nf[0], fxy = (int64_t)t.nf[0] * t.nf[1];
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  for (int fz = sz[2 * cz]; fz <= sz[2 * cz + 1]; ++fz) {
+    const float gz = axis_weight(lz, wz, fz, cz);
+    for (int fy = sy[2 * cy]; fy <= sy[2 * cy + 1]; ++fy) {
+      const float gyz = gz * axis_weight(ly, wy, fy, cy);
+      for (int fx = sx[2 * cx]; fx <= sx[2 * cx + 1]; ++fx) {
+        const float g = gyz * axis_weight(lx, wx, fx, cx);
+        const int64_t f = 3 * (fx + fx0 * fy + fxy * fz);
+        a0 += g * fine[f];
+        a1 += g * fine[f + 1];
+        a2 += g * fine[f + 2];
+      }
+    }
+  }
+  coarse[3 * I] = a0;
+  coarse[3 * I + 1] = a1;
+  coarse[3 * I + 2] = a2;
+}
+
+// fine[f] += sum over the (at most 8) coarse nodes I of P[f, I] coarse[I].
+__global__ void __launch_bounds__(kThreads)
+prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float* __restrict__ fine) {
+  const int64_t f = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  const int64_t nfx = t.nf[0], nfy = t.nf[1];
+  if (f >= nfx * nfy * t.nf[2]) return;
+  const int fx = (int)(f % nfx), fy = (int)((f / nfx) % nfy), fz = (int)(f / (nfx * nfy));
+  const int32_t* lx = t.left;
+  const int32_t* ly = lx + t.nf[0];
+  const int32_t* lz = ly + t.nf[1];
+  const float* wx = t.wright;
+  const float* wy = wx + t.nf[0];
+  const float* wz = wy + t.nf[1];
+  const int cx[2] = {lx[fx], lx[fx] + 1}, cy[2] = {ly[fy], ly[fy] + 1}, cz[2] = {lz[fz], lz[fz] + 1};
+  const float gx[2] = {1.0f - wx[fx], wx[fx]}, gy[2] = {1.0f - wy[fy], wy[fy]}, gz[2] = {1.0f - wz[fz], wz[fz]};
+  const int64_t ncx = t.nc[0], ncxy = (int64_t)t.nc[0] * t.nc[1];
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+  for (int kz = 0; kz < 2; ++kz) {
+    if (gz[kz] == 0.0f) continue;      // the right neighbour of a coinciding node (may be out of range)
+#pragma unroll
+    for (int ky = 0; ky < 2; ++ky) {
+      if (gy[ky] == 0.0f) continue;
+#pragma unroll
+      for (int kx = 0; kx < 2; ++kx) {
+        if (gx[kx] == 0.0f) continue;
+        const float g = gx[kx] * gy[ky] * gz[kz];
+        const int64_t I = 3 * (cx[kx] + ncx * cy[ky] + ncxy * cz[kz]);
+        a0 += g * coarse[I];
+        a1 += g * coarse[I + 1];
+        a2 += g * coarse[I + 2];
+      }
+    }
+  }
+  fine[3 * f] += a0;
+  fine[3 * f + 1] += a1;
+  fine[3 * f + 2] += a2;
+}
+
+// y = A x, A dense n x n row-major; one wave per row.
+__global__ void __launch_bounds__(kThreads)
+gemv_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__ x, float* __restrict__ y) {
+  const int64_t row = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  float s = 0.0f;
+  if (row < n) {
+    const float* __restrict__ a = A + row * n;
+    for (int64_t c = lane; c < n; c += 64) s += a[c] * x[c];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (row < n && lane == 0) y[row] = s;
+}
+
+int check_transfer(const fem_grid_transfer* t) {
+  FEM_REQUIRE(t && t->left && t->wright && t->span, FEM_E_INVALID, "null grid transfer table");
+  for (int a = 0; a < 3; ++a) {
+    FEM_REQUIRE(t->nf[a] >= 1 && t->nc[a] >= 1 && t->nc[a] <= t->nf[a], FEM_E_INVALID,
+                "grid transfer axis %d: %d fine, %d coarse nodes", a, t->nf[a], t->nc[a]);
+  }
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" int fem_mg_smooth(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+                             const float* x, const float* b, float* out, int mode, void* stream) {
+  FEM_REQUIRE(mesh && b && out, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(mode >= 0 && mode <= 2, FEM_E_INVALID, "mode must be 0 (residual), 1 (update) or 2 (start)");
+  FEM_REQUIRE(mode == 0 || dinv, FEM_E_INVALID, "null dinv");
+  FEM_REQUIRE(mode == 2 || (vals && x), FEM_E_INVALID, "null vals or x");
+  FEM_REQUIRE(mode != 1 || out != x, FEM_E_INVALID, "the update needs out != x (x is gathered)");
+  const int64_t N = mesh->num_node;
+  if (N == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  if (mode == 2) {
+    jacobi_start_kernel<<<ceil_div(N, kThreads), kThreads, 0, st>>>(N, dinv, omega, b, out);
+  } else if (mode == 0) {
+    smooth_kernel<0><<<ceil_div(N, kRowsPerBlock), kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+  } else {
+    smooth_kernel<1><<<ceil_div(N, kRowsPerBlock), kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+  }
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, void* stream) {
+  int rc = check_transfer(t);
+  if (rc) return rc;
+  FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
+  const int64_t Nc = (int64_t)t->nc[0] * t->nc[1] * t->nc[2];
+  restrict_kernel<<<ceil_div(Nc, kThreads), kThreads, 0, as_stream(stream)>>>(*t, fine, coarse);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, void* stream) {
+  int rc = check_transfer(t);
+  if (rc) return rc;
+  FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
+  const int64_t Nf = (int64_t)t->nf[0] * t->nf[1] * t->nf[2];
+  prolong_add_kernel<<<ceil_div(Nf, kThreads), kThreads, 0, as_stream(stream)>>>(*t, coarse, fine);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_dense_gemv_f32(const float* A, int64_t n, const float* x, float* y, void* stream) {
+  FEM_REQUIRE(A && x && y && n >= 0, FEM_E_INVALID, "null pointer or negative size");
+  FEM_REQUIRE(x != y, FEM_E_INVALID, "y must not alias x");
+  if (n == 0) return FEM_OK;
+  gemv_kernel<<<ceil_div(n, kThreads / 64), kThreads, 0, as_stream(stream)>>>(A, n, x, y);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}

@@ -1,0 +1,400 @@
+"""Geometric multigrid preconditioner for the static solve on structured grids.
+
+No counterpart in the reference, whose static solve is a Pardiso
+factorisation (SOLVE_static.py:31-38); this is one of the GPU forms that
+replace it, judged like the others by the FP64 residual ||F - K U|| / ||F|| on
+the penalised K.
+
+Hierarchy (symbolic, once per mesh, cached on the ``Physics``): the node grid
+``node_dims`` (x fastest) is coarsened by keeping every second index along
+each axis plus the last one, until at most ``min_nodes`` nodes remain or an
+axis has fewer than 3 nodes.  Each coarse level is a hex8 mesh of its own
+(the selected nodes at their fine coordinates, elements between neighbouring
+selected nodes) with its own GPU pattern, row groups and task lists, so its
+stiffness is assembled by the same kernels as the fine one
+(re-discretisation).  A coarse element's modulus is the mean of the fine
+moduli it covers; a coarse DOF is penalised when the coinciding fine DOF is.
+
+Numeric setup (per solve): FP32 K with the penalty on every level (FP64 on
+the coarsest), the 3x3 block-Jacobi inverses, a power-iteration estimate of
+lambda_max(Dinv K) per level giving the damping omega = 1 / theta of the
+degree-1 Chebyshev (damped block-Jacobi) smoother for the interval
+[lambda_max / 30, 1.1 lambda_max], and the dense inverse of the coarsest K.
+
+V-cycle (FP32, all on the GPU): per level x = omega Dinv b, r = b - K x,
+b_c = P^T r, recurse, x += P x_c, x <- x + omega Dinv (b - K x); the
+coarsest level applies its dense inverse.  The outer iteration is FP64
+flexible CG (FCG(1)): its stopping test looks at the residual of the free
+DOFs, because the penalised rows (diagonal 1e20) carry almost no energy and a
+CG iteration reduces their residual only by rounding luck; once the free part
+has converged, one block-Jacobi step on the penalised rows (exact to
+rounding, since their diagonal dominates) removes theirs, and the full FP64
+residual is checked before returning.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .runtime import device, stream_ptr
+
+CHEB_RATIO = 30.0      # smoothing interval [lambda_max / ratio, 1.1 lambda_max]
+CHEB_SAFETY = 1.1
+EIG_ITERS = 15
+
+
+def axis_select(n):
+    """Indices kept along an axis of n nodes: 0, 2, 4, ... and n - 1."""
+    sel = np.arange(0, n, 2)
+    if sel[-1] != n - 1:
+        sel = np.append(sel, n - 1)
+    return sel
+
+
+def axis_tables(n, sel):
+    """left (n,), wright (n,), span (2 nc,) of one axis (see fem_grid_transfer)."""
+    nc = len(sel)
+    f = np.arange(n)
+    left = np.searchsorted(sel, f, side="right") - 1
+    left = np.minimum(left, nc - 1)
+    nxt = np.minimum(left + 1, nc - 1)
+    gap = sel[nxt] - sel[left]
+    wright = np.where(gap > 0, (f - sel[left]) / np.maximum(gap, 1), 0.0)
+    first = np.where(np.arange(nc) > 0, np.concatenate([[0], sel[:-1] + 1]), 0)
+    last = np.where(np.arange(nc) < nc - 1, np.concatenate([sel[1:] - 1, [n - 1]]), n - 1)
+    span = np.stack([first, last], axis=1).reshape(-1)
+    return left.astype(np.int32), wright.astype(np.float32), span.astype(np.int32)
+
+
+def interpolation_matrix(dims_f, sels):
+    """Dense-free scipy form of P (nodes), for tests: kron(Pz, Py, Px)."""
+    import scipy.sparse as sp
+    mats = []
+    for n, sel in zip(dims_f, sels):
+        left, wright, _ = axis_tables(n, sel)
+        nc = len(sel)
+        rows = np.concatenate([np.arange(n), np.arange(n)])
+        cols = np.concatenate([left, np.minimum(left + 1, nc - 1)])
+        vals = np.concatenate([1.0 - wright, wright])
+        keep = vals != 0
+        mats.append(sp.csr_matrix((vals[keep], (rows[keep], cols[keep])), shape=(n, nc)))
+    return sp.kron(mats[2], sp.kron(mats[1], mats[0])).tocsr()
+
+
+def coarse_modulus(EX, dims_f, sels):
+    """Mean of the fine element moduli over each coarse element (scalar stays scalar)."""
+    ex = np.asarray(EX, dtype=np.float64)
+    if ex.ndim == 0:
+        return float(ex)
+    ne = [d - 1 for d in dims_f]
+    e3 = ex.reshape(ne[2], ne[1], ne[0])
+    for ax, sel in ((2, sels[0]), (1, sels[1]), (0, sels[2])):
+        starts = sel[:-1]
+        cnt = np.diff(sel).astype(np.float64)
+        e3 = np.add.reduceat(e3, starts, axis=ax)
+        shape = [1, 1, 1]
+        shape[ax] = -1
+        e3 = e3 / cnt.reshape(shape)
+    return np.ascontiguousarray(e3.reshape(-1))
+
+
+class Transfer:
+    """Device tables + the fem_grid_transfer descriptor between two grids."""
+
+    def __init__(self, dims_f, sels):
+        tabs = [axis_tables(n, s) for n, s in zip(dims_f, sels)]
+        d = device()
+        self.left = torch.as_tensor(np.concatenate([t[0] for t in tabs])).to(d)
+        self.wright = torch.as_tensor(np.concatenate([t[1] for t in tabs])).to(d)
+        self.span = torch.as_tensor(np.concatenate([t[2] for t in tabs])).to(d)
+        c = _lib.GridTransfer()
+        for a in range(3):
+            c.nf[a] = int(dims_f[a])
+            c.nc[a] = int(len(sels[a]))
+        c.left, c.wright, c.span = self.left.data_ptr(), self.wright.data_ptr(), self.span.data_ptr()
+        self.c = c
+        self.dims_f = tuple(int(x) for x in dims_f)
+        self.dims_c = tuple(len(s) for s in sels)
+
+    @property
+    def ref(self):
+        import ctypes
+        return ctypes.byref(self.c)
+
+
+def _fixed_dofs(model):
+    """(N, 3) bool: DOFs the penalty fixes (bound flag != -1)."""
+    N = model.Num_Node
+    fixed = np.zeros((N, 3), dtype=bool)
+    bn = model.bound_node.cpu().numpy().astype(np.int64)
+    fl = model.bound_flag.cpu().numpy().reshape(-1, 3)
+    for j in range(3):
+        fixed[bn[fl[:, j] != -1], j] = True
+    return fixed
+
+
+def _bounds_from_fixed(fixed):
+    """Bounds list (reference format) penalising exactly the DOFs of ``fixed``."""
+    code = fixed[:, 0] * 1 + fixed[:, 1] * 2 + fixed[:, 2] * 4
+    out = []
+    for c in range(1, 8):
+        ids = np.nonzero(code == c)[0]
+        if len(ids):
+            out.append([ids.astype(np.int64), [0.0 if (c >> j) & 1 else -1.0 for j in range(3)]])
+    return out
+
+
+class Hierarchy:
+    """The coarse grids of a structured mesh (symbolic; no matrix values)."""
+
+    def __init__(self, phys, fixed, min_nodes=1000, max_levels=12):
+        from .physics import Physics
+        from .mesh import grid_connectivity
+        if getattr(phys, "node_dims", None) is None:
+            raise ValueError("multigrid needs a structured node grid (Physics.node_dims)")
+        t0 = time.perf_counter()
+        dims = tuple(int(x) for x in phys.node_dims)
+        if int(np.prod(dims)) != phys.Num_Node:
+            raise ValueError("node_dims do not match the node count")
+        coords = np.ascontiguousarray(np.asarray(phys.Node[-3:], dtype=np.float64).T)
+        self.dims = [dims]
+        self.sels = []
+        self.transfers = []
+        self.phys = [phys]
+        self.node_maps = []             # coarse node -> node of the finer level
+        self.fixed = [fixed]
+        while len(self.dims) < max_levels:
+            d = self.dims[-1]
+            if int(np.prod(d)) <= min_nodes or min(d) < 3:
+                break
+            sels = [axis_select(n) for n in d]
+            dc = tuple(len(s) for s in sels)
+            idx = (sels[0][None, None, :] + d[0] * (sels[1][None, :, None] + d[1] * sels[2][:, None, None])).ravel()
+            coords = coords[idx]
+            fixed_c = self.fixed[-1][idx]
+            Nc = coords.shape[0]
+            Node = np.concatenate([np.arange(1, Nc + 1, dtype=np.float64)[None, :], coords.T], axis=0)
+            Elem = grid_connectivity(dc[0] - 1, dc[1] - 1, dc[2] - 1).T.copy()
+            pc = Physics(Node, Elem, _bounds_from_fixed(fixed_c), [], name=f"{phys.name}_mg{len(self.dims)}",
+                         node_dims=dc)
+            self.sels.append(sels)
+            self.transfers.append(Transfer(d, sels))
+            self.node_maps.append(idx)
+            self.dims.append(dc)
+            self.phys.append(pc)
+            self.fixed.append(fixed_c)
+        self.seconds = time.perf_counter() - t0
+
+    @property
+    def num_levels(self):
+        return len(self.dims)
+
+
+def hierarchy_for(model, min_nodes=1000):
+    """The cached hierarchy of the model's mesh (built on first use)."""
+    phys = model.Physics
+    key = ("mg_hierarchy", min_nodes)
+    cache = phys.__dict__.setdefault("_mg_cache", {})
+    if key not in cache:
+        cache[key] = Hierarchy(phys, _fixed_dofs(model), min_nodes=min_nodes)
+    return cache[key]
+
+
+class MultigridPreconditioner:
+    """FP32 V-cycle z = MG(r) for FP64 r (numeric setup in the constructor)."""
+
+    def __init__(self, model, hier: Hierarchy, seed=0):
+        from .model import Model
+        from .solvers import BsrOperator
+        t0 = time.perf_counter()
+        d = device()
+        self.hier = hier
+        L = hier.num_levels
+        self.meshes, self.k32, self.dinv, self.omega = [], [], [], []
+        EX = model.EX
+        mats = {"EX": EX, "PO": model.PO, "DE": model.DE}
+        for l in range(L):
+            if l == 0:
+                m = model
+            else:
+                mats = {"EX": coarse_modulus(mats["EX"], hier.dims[l - 1], hier.sels[l - 1]), "PO": model.PO,
+                        "DE": model.DE}
+                m = Model(hier.phys[l].name, hier.phys[l], mats, Omegas=(0.0, 0.0, 0.0))
+            self.meshes.append(m.dev)
+            if l < L - 1:
+                k32 = m.K_bsr("f32") if l == 0 else m.assemble_K("f32", "bsr")
+                op = BsrOperator(m.dev, k32)
+                dinv, _ = op.block_jacobi()
+                self.k32.append(k32)
+                self.dinv.append(dinv)
+                self.omega.append(1.0 / self._theta(m.dev, k32, dinv, seed + l))
+            else:
+                self.k_coarse = m.assemble_K("f64", "bsr")
+                self.coarse_inv = self._dense_inverse(m, self.k_coarse)
+        n = [3 * int(np.prod(dm)) for dm in hier.dims]
+        self.b = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
+        self.x = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
+        self.t = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
+        self.applications = 0
+        torch.cuda.synchronize()
+        self.setup_seconds = time.perf_counter() - t0
+
+    @staticmethod
+    def _theta(dev, k32, dinv, seed):
+        """Centre of the Chebyshev interval from lambda_max(Dinv K) by power iteration."""
+        st = stream_ptr()
+        n = 3 * dev.num_node
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        x = torch.randn(n, generator=g, dtype=torch.float32).to(device())
+        x /= torch.linalg.vector_norm(x)
+        zero = torch.zeros_like(x)
+        r = torch.empty_like(x)
+        y = torch.empty_like(x)
+        lam = torch.zeros((), dtype=torch.float32, device=x.device)
+        for _ in range(EIG_ITERS):
+            _lib.call("fem_mg_smooth", dev.ref, k32.data_ptr(), dinv.data_ptr(), 1.0, x.data_ptr(), zero.data_ptr(),
+                      r.data_ptr(), 0, st)                       # r = -K x
+            _lib.call("fem_mg_smooth", dev.ref, None, dinv.data_ptr(), -1.0, None, r.data_ptr(), y.data_ptr(), 2,
+                      st)                                        # y = Dinv K x
+            lam = torch.linalg.vector_norm(y)
+            x = y / lam
+        lmax = float(lam)
+        if not (math.isfinite(lmax) and lmax > 0):
+            raise _lib.FemError(_lib.FEM_E_NUMERIC, f"multigrid: lambda_max(Dinv K) estimate {lmax}")
+        hi, lo = CHEB_SAFETY * lmax, lmax / CHEB_RATIO
+        return 0.5 * (hi + lo)
+
+    @staticmethod
+    def _dense_inverse(m, vals64):
+        """FP32 copy of the inverse of the coarsest (penalised) K, by FP64 Cholesky."""
+        N = m.Num_Node
+        d = device()
+        co = m.dev["co_node"].long()
+        rows = torch.repeat_interleave(torch.arange(N, device=d), co)
+        cols = m.dev["re_node"].long()
+        r3 = (3 * rows[:, None, None] + torch.arange(3, device=d)[None, :, None]).expand(-1, 3, 3)
+        c3 = (3 * cols[:, None, None] + torch.arange(3, device=d)[None, None, :]).expand(-1, 3, 3)
+        A = torch.zeros(3 * N, 3 * N, dtype=torch.float64, device=d)
+        A[r3.reshape(-1), c3.reshape(-1)] = vals64.view(-1)
+        Lc, info = torch.linalg.cholesky_ex(A)
+        if int(info) != 0:
+            raise _lib.FemError(_lib.FEM_E_NUMERIC, "multigrid: the coarsest K is not positive definite")
+        inv = torch.cholesky_inverse(Lc)
+        return inv.float().contiguous()
+
+    def vcycle(self, b0: torch.Tensor) -> torch.Tensor:
+        """x = MG(b0) in FP32 (b0 FP32 on the finest level)."""
+        st = stream_ptr()
+        L = self.hier.num_levels
+        self.b[0] = b0
+        for l in range(L - 1):
+            dev = self.meshes[l]
+            k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
+            _lib.call("fem_mg_smooth", dev.ref, None, dv, w, None, self.b[l].data_ptr(), self.x[l].data_ptr(), 2, st)
+            _lib.call("fem_mg_smooth", dev.ref, k, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
+                      self.t[l].data_ptr(), 0, st)
+            _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, self.t[l].data_ptr(), self.b[l + 1].data_ptr(), st)
+        n = self.b[L - 1].numel()
+        _lib.call("fem_dense_gemv_f32", self.coarse_inv.data_ptr(), n, self.b[L - 1].data_ptr(),
+                  self.x[L - 1].data_ptr(), st)
+        for l in range(L - 2, -1, -1):
+            dev = self.meshes[l]
+            k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
+            _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, self.x[l + 1].data_ptr(), self.x[l].data_ptr(),
+                      st)
+            _lib.call("fem_mg_smooth", dev.ref, k, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
+                      self.t[l].data_ptr(), 1, st)
+            self.x[l], self.t[l] = self.t[l], self.x[l]
+        return self.x[0]
+
+    def __call__(self, r: torch.Tensor) -> torch.Tensor:
+        self.applications += 1
+        return self.vcycle(r.float().contiguous()).double()
+
+
+def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, drop=1e-2):
+    """FP64 flexible CG (FCG(1)) with a nonlinear or inexact preconditioner.
+
+    Stops when the free-DOF residual is below tol ||F|| and, after one
+    block-Jacobi correction on the penalised rows, the full FP64 residual is
+    too; replaces the recursive residual by F - K x whenever it has dropped by
+    ``drop``.  Returns (x, info dict)."""
+    from .learned import free_dof_mask
+    free = free_dof_mask(model)
+    fixed = 1.0 - free
+    dinv64, _ = K64.block_jacobi()
+    d3 = dinv64.view(-1, 3, 3)
+    bn = float(torch.linalg.vector_norm(F))
+    x = torch.zeros_like(F)
+    r = F.clone()
+    if bn == 0.0:
+        return x, {"iterations": 0, "rel_residual": 0.0, "converged": True}
+    p_prev = q_prev = pq_prev = None
+    anchor = bn
+    it = 0
+    rel = 1.0
+    converged = False
+    trace = []
+    while it < maxiter:
+        z = precond(r)
+        if p_prev is None:
+            p = z
+        else:
+            p = z - (torch.dot(z, q_prev) / pq_prev) * p_prev
+        q = K64.matvec(p)
+        pq = torch.dot(p, q)
+        alpha = torch.dot(p, r) / pq
+        x.add_(alpha * p)
+        r.sub_(alpha * q)
+        p_prev, q_prev, pq_prev = p, q, pq
+        it += 1
+        if it % check_every:
+            continue
+        rf = float(torch.linalg.vector_norm(r * free))
+        if not math.isfinite(rf):
+            break
+        trace.append((it, rf / bn))
+        if rf < drop * anchor or rf < tol * bn:
+            r = F - K64.matvec(x)
+            anchor = float(torch.linalg.vector_norm(r * free))
+            if anchor < tol * bn:
+                x.add_(torch.bmm(d3, r.view(-1, 3, 1)).view(-1) * fixed)
+                r = F - K64.matvec(x)
+                rel = float(torch.linalg.vector_norm(r)) / bn
+                if rel < tol:
+                    converged = True
+                    break
+    if not converged:
+        rel = float(torch.linalg.vector_norm(F - K64.matvec(x))) / bn
+    return x, {"iterations": it, "rel_residual": rel, "converged": converged, "trace": trace}
+
+
+def solve_mg(model, K64, F, tol=1e-8, maxiter=1000, min_nodes=1000, fallback=True, **kw):
+    """Static solve: FP64 flexible CG preconditioned by the FP32 multigrid
+    V-cycle; falls back to FP64 block-Jacobi PCG from the best iterate when
+    it does not reach ``tol`` within ``maxiter``."""
+    from . import solvers
+    t0 = time.perf_counter()
+    hier = hierarchy_for(model, min_nodes=min_nodes)
+    t1 = time.perf_counter()
+    pre = MultigridPreconditioner(model, hier)
+    x, info = solve_fcg_mg(model, K64, F, pre, tol=tol, maxiter=maxiter, **kw)
+    info["levels"] = [tuple(dm) for dm in hier.dims]
+    info["hierarchy_seconds"] = hier.seconds          # symbolic, once per mesh
+    info["hierarchy_built_now"] = t1 - t0 > 0.5 * hier.seconds
+    info["setup_seconds"] = pre.setup_seconds
+    info["fell_back"] = False
+    if not info["converged"] and fallback:
+        info["fell_back"] = True
+        dinv64, _ = K64.block_jacobi()
+        x, it2, _ = solvers.pcg(K64, F, dinv64, tol, 200000, x0=x.clone())
+        x = x.clone()
+        info["iterations"] += it2
+        info["rel_residual"] = float(torch.linalg.vector_norm(F - K64.matvec(x)) / torch.linalg.vector_norm(F))
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t1
+    return x, solvers.SolveInfo("mg", info["iterations"], 1, info["rel_residual"], seconds, [info])

@@ -2,10 +2,11 @@
 
 ``Static_solver(Model, 0).Solve_cpu()`` keeps the reference's name and
 return value (U as an ndarray (3N,)).  The Pardiso factorisation is replaced
-by one of the GPU forms of solvers.py, chosen by the extra keyword
-``method`` ('pcg' FP64 block-Jacobi PCG, 'mixed' FP32-PCG inside FP64
-refinement, 'learned' flexible PCG with the mesh-convolution
-preconditioner).  ``tol`` is the FP64 relative residual target; the achieved
+by one of the GPU forms, chosen by the extra keyword ``method`` ('pcg' FP64
+block-Jacobi PCG, 'mixed' FP32-PCG inside FP64 refinement, 'mg' FP64
+flexible CG preconditioned by an FP32 geometric multigrid V-cycle on
+structured grids (multigrid.py), 'learned' flexible PCG with the
+mesh-convolution preconditioner).  ``tol`` is the FP64 relative residual target; the achieved
 value is printed as the reference prints its Pardiso residual and kept in
 ``self.info``.
 
@@ -47,6 +48,9 @@ class Static_solver:
         elif method == "learned":
             from .learned import solve_learned
             U, info = solve_learned(M, K64, F, tol=tol, **kw)
+        elif method == "mg":
+            from .multigrid import solve_mg
+            U, info = solve_mg(M, K64, F, tol=tol, **kw)
         else:
             raise ValueError(f"unknown method {method!r}")
         self.info = info

@@ -243,6 +243,39 @@ int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const double* din
                      double* X, double* R, double* Z, double* P, double* Q, double* state,
                      void* work, int64_t work_bytes, void* stream);
 
+/* ---- geometric multigrid V-cycle on structured node grids (FP32), the
+ *      preconditioner of the FP64 flexible CG static solve that replaces the
+ *      Pardiso call of SOLVE_static.py:31-38; no counterpart in the
+ *      reference) -------------------------------------------------------------
+ * Node grids are numbered x fastest, n = ix + nf[0]*(iy + nf[1]*iz).  The
+ * coarse grid keeps every second index along each axis plus the last one;
+ * prolongation P is trilinear in the index coordinate, restriction is P^T.
+ * Per axis a (tables concatenated over a = 0, 1, 2):
+ *   left[f]    coarse index at or left of fine index f,
+ *   wright[f]  weight of coarse index left[f] + 1 (0 when f coincides with
+ *              a coarse node; then left[f] + 1 is never read),
+ *   span[2c], span[2c+1]  first and last fine index with a nonzero weight
+ *              for coarse index c. */
+typedef struct fem_grid_transfer {
+  int32_t nf[3];
+  int32_t nc[3];
+  const int32_t* left;    /* nf[0] + nf[1] + nf[2] */
+  const float* wright;    /* nf[0] + nf[1] + nf[2] */
+  const int32_t* span;    /* 2 * (nc[0] + nc[1] + nc[2]) */
+} fem_grid_transfer;
+
+/* FP32 block-Jacobi smoother pieces on the BSR(3x3) FP32 matrix vals:
+ * mode 0: out = b - K x;  mode 1: out = x + omega Dinv (b - K x) (out must
+ * not alias x);  mode 2: out = omega Dinv b (vals and x unused). */
+int fem_mg_smooth(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+                  const float* x, const float* b, float* out, int mode, void* stream);
+/* coarse = P^T fine (3 FP32 values per node). */
+int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, void* stream);
+/* fine += P coarse. */
+int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, void* stream);
+/* y = A x, A dense n x n FP32 row-major (the coarsest level's inverse). */
+int fem_dense_gemv_f32(const float* A, int64_t n, const float* x, float* y, void* stream);
+
 /* ---- 3x3x3 mesh convolution on the structured node grid (the learned
  *      preconditioner; no counterpart in the reference) ---------------------
  * BF16 operands on MFMA, FP32 accumulation.  Activations channels-last

@@ -46,3 +46,13 @@ def test_fem_mesh_struct_matches_header():
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = re.findall(r"(\w+)\s*;", body)
     assert fields == [f[0] for f in _lib.FemMesh._fields_]
+
+
+def test_grid_transfer_struct_matches_header():
+    """Field names and order of the ctypes mirror equal struct fem_grid_transfer's."""
+    from fem_taichi_amd import _lib
+    text = open(HEADER).read()
+    body = re.search(r"typedef struct fem_grid_transfer \{(.*?)\} fem_grid_transfer;", text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"(\w+)\s*(?:\[\d+\])?\s*;", body)
+    assert fields == [f[0] for f in _lib.GridTransfer._fields_]
