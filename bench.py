@@ -16,8 +16,12 @@ The JSON line also carries:
                    (oracle/assemble_ref.c) on the same mesh, median of 5 after
                    one warm-up run, on this host's cores;
   f32           -- the same assembly into an FP32 BSR copy;
-  static_solve  -- (with --solve) seconds to an FP64-verified 1e-8 relative
-                   residual on the rotating twisted blade, per solver form.
+  static_solve  -- seconds to an FP64-verified 1e-8 relative residual on the
+                   rotating twisted blade 400x50x50 (1,040,451 nodes), per
+                   solver form: FP64 block-Jacobi PCG, FP64 flexible CG with
+                   the FP32 multigrid V-cycle, and the learned form (the
+                   V-cycle plus the mesh-convolution correction), with the
+                   speed-up of each over plain PCG (--no-solve skips them).
 """
 from __future__ import annotations
 
@@ -40,7 +44,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--solve", action="store_true", help="also time the static solver forms")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the static solves (plain PCG, multigrid, learned multigrid) on the blade")
+    ap.add_argument("--solve-all", action="store_true",
+                    help="also time the FP32-matrix PCG and the block-Jacobi learned forms")
     ap.add_argument("--solve-mesh", default="400x50x50")
     ap.add_argument("--modal", default=None, metavar="NXxNYxNZ:MODES:BLOCK",
                     help="also time block Lanczos (Eigen_solver) on a twisted blade, e.g. 400x50x50:20:8")
@@ -96,9 +103,18 @@ def cpu_baseline(grid, phys):
                                       f"({med:.3f} s each) after 1 warm-up"}
 
 
-def time_solves(mesh_spec):
+def time_solves(mesh_spec, all_forms=False):
+    """Seconds to an FP64-verified 1e-8 relative residual on the twisted
+    blade, per solver form.  Every form runs twice on the assembled K and F
+    (inputs resident in HBM); ``seconds`` is the second run and
+    ``seconds_first_call`` the first, which also pays one-time library
+    initialisation.  The multigrid forms' coarse grids, patterns and task
+    lists (symbolic, once per mesh, like the fine pattern) are timed apart as
+    ``hierarchy_seconds``; their numeric setup (coarse assembly, block-Jacobi
+    inverses, eigenvalue estimates, coarsest inverse) is inside ``seconds``."""
     import torch
-    from fem_taichi_amd import Model, Physics, mesh, solvers
+    from fem_taichi_amd import Model, Physics, mesh, solvers, multigrid
+    from fem_taichi_amd.learned import solve_learned
     nx, ny, nz = (int(v) for v in mesh_spec.split("x"))
     g = mesh.twisted_blade(nx, ny, nz)
     phys = Physics.from_grid(g)
@@ -108,24 +124,38 @@ def time_solves(mesh_spec):
     K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32"))
     torch.cuda.synchronize()
     out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "tol": 1e-8}
-    _, info = solvers.solve_pcg64(K64, F, tol=1e-8)
-    out["pcg_fp64_bj"] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual}
-    print(f"# pcg_fp64_bj {json.dumps(out['pcg_fp64_bj'])}", flush=True)
-    _, info = solvers.solve_mixed(K64, K32, F, tol=1e-8)
-    out["mixed_fp32_pcg_fp64_refine"] = {"seconds": info.seconds, "iterations": info.iterations,
-                                         "outer": info.outer, "rel_residual": info.rel_residual}
-    print(f"# mixed {json.dumps(out['mixed_fp32_pcg_fp64_refine'])}", flush=True)
-    from fem_taichi_amd.learned import solve_learned
-    for key, use_net, matrix in (("learned_fcg_f32_matrix", True, "f32"), ("learned_fcg_f64_matrix", True, "f64"),
-                                 ("fcg_f64_matrix_bj_only", False, "f64")):
-        # the flexible iteration runs to tol unless it diverges or its FP64
-        # residual stalls (no plateau test), so the counts compare the
-        # preconditioners themselves; FP64 PCG finishes only after a stall
-        _, info = solve_learned(model, K64, F, tol=1e-8, use_net=use_net, matrix=matrix, stagnation=None,
-                                maxiter=20000)
-        out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual,
-                    "fell_back": bool(info.history[0]["fell_back"])}
+    t0 = time.perf_counter()
+    hier = multigrid.hierarchy_for(model)
+    out["hierarchy_seconds"] = time.perf_counter() - t0
+    out["levels"] = [list(d) for d in hier.dims]
+
+    def run(key, fn, extra=lambda info: {}):
+        first = fn()
+        info = fn()
+        out[key] = {"seconds": info.seconds, "seconds_first_call": first.seconds, "iterations": info.iterations,
+                    "rel_residual": info.rel_residual, **extra(info)}
         print(f"# {key} {json.dumps(out[key])}", flush=True)
+
+    run("pcg_fp64_bj", lambda: solvers.solve_pcg64(K64, F, tol=1e-8)[1])
+    run("mg_fp32_vcycle_fp64_fcg", lambda: multigrid.solve_mg(model, K64, F, tol=1e-8)[1],
+        lambda info: {"setup_seconds": info.history[0]["setup_seconds"]})
+    run("learned_mg", lambda: solve_learned(model, K64, F, tol=1e-8, base="mg")[1],
+        lambda info: {"fell_back": bool(info.history[0]["fell_back"])})
+    if all_forms:
+        run("mixed_fp32_pcg_fp64_refine", lambda: solvers.solve_mixed(K64, K32, F, tol=1e-8)[1],
+            lambda info: {"outer": info.outer})
+        for key, use_net, matrix in (("learned_bj_fcg_f32_matrix", True, "f32"),
+                                     ("fcg_f64_matrix_bj_only", False, "f64")):
+            # the flexible iteration runs to tol unless it diverges or its FP64
+            # residual stalls (no plateau test); FP64 PCG finishes after a stall
+            info = solve_learned(model, K64, F, tol=1e-8, use_net=use_net, matrix=matrix, stagnation=None,
+                                 maxiter=20000, base="bj")[1]
+            out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual,
+                        "fell_back": bool(info.history[0]["fell_back"])}
+            print(f"# {key} {json.dumps(out[key])}", flush=True)
+    base = out["pcg_fp64_bj"]["seconds"]
+    out["speedup_vs_pcg_fp64_bj"] = {k: base / v["seconds"] for k, v in out.items()
+                                     if isinstance(v, dict) and "seconds" in v and k != "pcg_fp64_bj"}
     print(f"# solves {json.dumps(out)}", flush=True)
     return out
 
@@ -238,8 +268,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(g, phys)
-    if args.solve:
-        result["static_solve"] = time_solves(args.solve_mesh)
+    if not args.no_solve:
+        result["static_solve"] = time_solves(args.solve_mesh, all_forms=args.solve_all)
     if args.modal:
         result["modal"] = time_modal(args.modal)
     if args.copy_probe:

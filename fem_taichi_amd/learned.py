@@ -198,16 +198,26 @@ def free_dof_mask(model):
 
 
 class LearnedPreconditioner:
-    """z = Dinv r + s * net(r / s, log E) / E_REF with s = RMS(r)."""
+    """z = Dinv r + s * net(r / s, log E) / E_REF with s = RMS(r) (block-Jacobi
+    base), or, with a ``base`` preconditioner B (the multigrid V-cycle):
+    z = B r + tau c, c = s * net(r_a / s, log E) / E_REF for the residual
+    r_a = r - K B r the base leaves, and tau = c.r_a / c.K c, the step that
+    minimises the K-norm of the error along c (so the learned term can only
+    lower it)."""
 
-    def __init__(self, net: MeshConvNet, node_dims, efeat, dinv64, free_mask=None, use_net=True):
+    def __init__(self, net: MeshConvNet, node_dims, efeat, dinv64, free_mask=None, use_net=True, base=None, K=None):
         self.net = net
         self.grid = ConvWork(*node_dims)
         self.efeat = efeat
         self.dinv = dinv64.view(-1, 3, 3)
         self.mask = free_mask        # (3N,) 1 on free DOFs, 0 on penalised ones
         self.use_net = use_net
+        self.base = base
+        self.K = K
+        if base is not None and K is None:
+            raise ValueError("a base preconditioner needs the operator K for its residual")
         self.applications = 0
+        self.taus = []
 
     def features(self, r):
         P = self.grid.P
@@ -222,10 +232,20 @@ class LearnedPreconditioner:
 
     def __call__(self, r):
         self.applications += 1
-        z = self.bj(r)
+        if self.base is None:
+            z = self.bj(r)
+            if not self.use_net:
+                return z
+            return z + self.correction(r)
+        z = self.base(r)
         if not self.use_net:
             return z
-        return z + self.correction(r)
+        ra = r - self.K.matvec(z)
+        c = self.correction(ra)
+        kc = self.K.matvec(c)
+        tau = torch.nan_to_num(torch.dot(c, ra) / torch.dot(c, kc), nan=0.0, posinf=0.0, neginf=0.0)
+        self.taus.append(tau)
+        return z + tau * c
 
     def correction(self, r, keep=False):
         x0, s = self.features(r)
@@ -333,33 +353,64 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
 
 
 def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=100000, use_net=True,
-                  matrix="f32", **kw):
+                  matrix="f32", base="mg", **kw):
     """Static solve with the learned preconditioner (structured meshes).
-    ``matrix``: "f32" streams the FP32 copy of K in the flexible iteration,
-    "f64" the FP64 one."""
+
+    base "mg": flexible CG (multigrid.solve_fcg_mg) preconditioned by the FP32
+    multigrid V-cycle plus the learned correction of the residual it leaves
+    (weights of the "mg" role); when that does not reach ``tol`` in
+    ``maxiter`` iterations, FP64 block-Jacobi PCG finishes from its iterate.
+    base "bj": truncated flexible CG (solve_fcg) with block-Jacobi plus the
+    learned correction (weights of the "bj" role); ``matrix`` "f32" streams
+    the FP32 copy of K in that iteration, "f64" the FP64 one."""
     from . import solvers
     phys = model.Physics
     if getattr(phys, "node_dims", None) is None:
         raise ValueError("the learned preconditioner needs a structured node grid (Physics.node_dims)")
+    if base not in ("mg", "bj"):
+        raise ValueError(f"unknown base {base!r}")
     t0 = time.perf_counter()
     if net is None:
         net = MeshConvNet()
         if weights is None:
-            weights = default_weights_path()
+            weights = default_weights_path(base)
         net.load_state_dict(torch.load(weights, weights_only=True))
-    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32")) if matrix == "f32" else None
     dinv64, _ = K64.block_jacobi()
-    pre = LearnedPreconditioner(net, phys.node_dims, node_log_modulus(phys, model.EX), dinv64,
-                                free_mask=free_dof_mask(model), use_net=use_net)
+    efeat = node_log_modulus(phys, model.EX)
+    if base == "mg":
+        from . import multigrid
+        hier = multigrid.hierarchy_for(model)
+        t0 = time.perf_counter()                # the symbolic hierarchy is per-mesh preprocessing
+        mg = multigrid.MultigridPreconditioner(model, hier)
+        pre = LearnedPreconditioner(net, phys.node_dims, efeat, dinv64, free_mask=free_dof_mask(model),
+                                    use_net=use_net, base=mg, K=K64)
+        x, info = multigrid.solve_fcg_mg(model, K64, F, pre, tol=tol, maxiter=min(maxiter, 1000), **kw)
+        info["fell_back"] = False
+        info["applications"] = pre.applications
+        info["taus"] = [float(v) for v in pre.taus[:64]]
+        if not info["converged"]:
+            info["fell_back"] = True
+            x, it2, _ = solvers.pcg(K64, F, dinv64, tol, maxiter, x0=x.clone())
+            x = x.clone()
+            info["iterations"] += it2
+            info["rel_residual"] = float(torch.linalg.vector_norm(F - K64.matvec(x)) /
+                                         torch.linalg.vector_norm(F))
+        torch.cuda.synchronize()
+        return x, solvers.SolveInfo("learned_mg", info["iterations"], 1, info["rel_residual"],
+                                    time.perf_counter() - t0, [info])
+    K32 = solvers.BsrOperator(model.dev, model.K_bsr("f32")) if matrix == "f32" else None
+    pre = LearnedPreconditioner(net, phys.node_dims, efeat, dinv64, free_mask=free_dof_mask(model), use_net=use_net)
     x, info = solve_fcg(K64, K32, F, pre, tol=tol, maxiter=maxiter, **kw)
     torch.cuda.synchronize()
     return x, solvers.SolveInfo("learned", info["iterations"], 1, info["rel_residual"],
                                 time.perf_counter() - t0, [info])
 
 
-def default_weights_path():
+def default_weights_path(role="bj"):
+    """Shipped weights: trained beside block-Jacobi ("bj") or after the multigrid V-cycle ("mg")."""
     import os
-    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "meshconv_slab.pt")
+    name = {"bj": "meshconv_slab.pt", "mg": "meshconv_mg.pt"}[role]
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", name)
 
 
 # ---------------------------------------------------------------------------
@@ -392,12 +443,21 @@ def pcg_trajectory(K64, b, dinv64, snaps, maxiter):
 
 class TrainingSlab:
     """One training slab on the device: K (FP64 BSR), the preconditioner
-    pieces and residual snapshots r_k of plain block-Jacobi PCG runs."""
+    pieces and residual snapshots.
 
-    def __init__(self, grid_mesh, snaps=(0, 3, 10, 30, 100, 300, 1000), rhs_seeds=(0,), with_physical=True):
+    role "bj": the residuals r_k of plain block-Jacobi PCG runs at the
+    iterations ``snaps``; the network learns the correction added to Dinv r.
+    role "mg": the residuals r_a = r_k - K MG(r_k) that the multigrid V-cycle
+    leaves at the iterations ``snaps`` of multigrid flexible CG runs; the
+    network learns the correction added to MG(r_k)."""
+
+    def __init__(self, grid_mesh, snaps=None, rhs_seeds=(0,), with_physical=True, role="bj"):
         from . import solvers
         from .model import Model
         from .physics import Physics
+        if role not in ("bj", "mg"):
+            raise ValueError(f"unknown role {role!r}")
+        self.role = role
         self.phys = Physics.from_grid(grid_mesh)
         self.model = Model(grid_mesh.name, self.phys, grid_mesh.Materials, Omegas=grid_mesh.Omegas)
         self.K64 = solvers.BsrOperator(self.model.dev, self.model.K_bsr("f64"))
@@ -410,24 +470,73 @@ class TrainingSlab:
             g = torch.Generator(device="cpu").manual_seed(int(sd))
             rhs.append(torch.randn(n, generator=g, dtype=torch.float64).to(device()))
         self.residuals = []
-        for b in rhs:
-            traj = pcg_trajectory(self.K64, b, dinv64, snaps, max(snaps))
-            self.residuals += [rk for _, (_, rk) in sorted(traj.items())]
+        if role == "bj":
+            snaps = snaps or (0, 3, 10, 30, 100, 300, 1000)
+            for b in rhs:
+                traj = pcg_trajectory(self.K64, b, dinv64, snaps, max(snaps))
+                self.residuals += [rk for _, (_, rk) in sorted(traj.items())]
+            return
+        from . import multigrid
+        snaps = set(snaps or (0, 1, 2, 4, 7, 11, 16))
+        mg = multigrid.MultigridPreconditioner(self.model, multigrid.hierarchy_for(self.model))
+        count = [0]
 
-    def loss_and_grad(self, net, r):
-        """loss = ||r - K z||^2 / ||r||^2 and dloss/d(net output) (P, 16)."""
+        def recording(r):
+            z = mg(r)
+            if count[0] in snaps:
+                self.residuals.append((r - self.K64.matvec(z)).contiguous())
+            count[0] += 1
+            return z
+
+        for b in rhs:
+            count[0] = 0
+            multigrid.solve_fcg_mg(self.model, self.K64, b, recording, tol=1e-12, maxiter=max(snaps) + 1)
+        # the energy e.K.e = e.r_a of each residual's error e = K^-1 r_a (tight multigrid solves)
+        self.energies = []
+        for ra in self.residuals:
+            e, info = multigrid.solve_fcg_mg(self.model, self.K64, ra, mg, tol=1e-10, maxiter=200)
+            self.energies.append(float(torch.dot(e, ra)))
+
+    def loss_and_grad(self, net, r, energy=None):
+        """The loss and dloss/d(net output) (P, 16).
+
+        role "bj": loss = ||r - K z||^2 / ||r||^2 with z = Dinv r + c.
+        role "mg": loss = min over tau of ||e - tau c||_K^2 / ||e||_K^2
+        = 1 - (c.r)^2 / ((c.K c) (e.r)) for the error e = K^-1 r of the
+        residual r the V-cycle leaves (only the scalar e.r = ``energy`` is
+        needed): the share of the error's energy that the correction, taken
+        with the step the preconditioner uses, leaves behind.  Scale-free in
+        c, and measured in the norm flexible CG minimises."""
         self.pre.net = net
-        zc = self.pre.bj(r)
         x0, s = self.pre.features(r)
         y = net.forward(x0, self.pre.grid, keep=True)
         c = y[:, :3].reshape(-1).double() * (s / E_REF) * self.pre.mask
-        res = r - self.K64.matvec((zc + c).contiguous())
-        rn = torch.dot(r, r)
-        loss = torch.dot(res, res) / rn
-        g = self.K64.matvec(res.contiguous()) * (-2.0 / rn)          # dloss/dz (K symmetric)
+        if self.role == "bj":
+            z = self.pre.bj(r) + c
+            res = r - self.K64.matvec(z.contiguous())
+            rn = torch.dot(r, r)
+            loss = torch.dot(res, res) / rn
+            g = self.K64.matvec(res.contiguous()) * (-2.0 / rn)          # dloss/dz (K symmetric)
+        else:
+            if energy is None or not energy > 0:
+                raise ValueError("role 'mg' needs the residual's error energy e.r > 0")
+            kc = self.K64.matvec(c.contiguous())
+            a = torch.dot(c, r)
+            b = torch.dot(c, kc).clamp_min(1e-300)
+            loss = 1.0 - a * a / (b * energy)
+            g = (r - (a / b) * kc) * (-2.0 * a / (b * energy))            # dloss/dc
         dy = torch.zeros_like(y)
         dy[:, :3] = (g * self.pre.mask * (s / E_REF)).view(-1, 3).float()
         return loss, dy
+
+    def samples(self):
+        """(residual, energy) pairs (energy None for role "bj")."""
+        if self.role == "bj":
+            return [(r, None) for r in self.residuals]
+        return list(zip(self.residuals, self.energies))
+
+    def mean_loss(self, net):
+        return float(np.mean([float(self.loss_and_grad(net, r, en)[0]) for r, en in self.samples()]))
 
 
 def train(net, slabs, steps=400, lr=1e-3, seed=0, log_every=50, log=None):
@@ -437,11 +546,11 @@ def train(net, slabs, steps=400, lr=1e-3, seed=0, log_every=50, log=None):
         w.requires_grad_(True)
     opt = torch.optim.Adam(net.w + net.b, lr=lr)
     rng = np.random.default_rng(seed)
-    flat = [(sl, r) for sl in slabs for r in sl.residuals]
+    flat = [(sl, r, en) for sl in slabs for r, en in sl.samples()]
     hist = []
     for step in range(steps):
-        sl, r = flat[rng.integers(len(flat))]
-        loss, dy = sl.loss_and_grad(net, r)
+        sl, r, en = flat[rng.integers(len(flat))]
+        loss, dy = sl.loss_and_grad(net, r, en)
         gw, gb = net.backward(dy)
         gw[0][:, :, 4:] = 0
         gw[2][:, 3:, :] = 0
@@ -458,7 +567,10 @@ def train(net, slabs, steps=400, lr=1e-3, seed=0, log_every=50, log=None):
 
 
 def bj_loss(slab):
-    """The same loss with the learned correction switched off (block-Jacobi)."""
+    """The same loss with the learned correction switched off (block-Jacobi;
+    1 for role "mg", where the residuals are already the base's)."""
+    if slab.role == "mg":
+        return 1.0
     vals = []
     for r in slab.residuals:
         res = r - slab.K64.matvec(slab.pre.bj(r).contiguous())
