@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--solve-mesh", default="400x50x50")
     ap.add_argument("--modal", default=None, metavar="NXxNYxNZ:MODES:BLOCK",
                     help="also time block Lanczos (Eigen_solver) on a twisted blade, e.g. 400x50x50:20:8")
+    ap.add_argument("--modal-inner", choices=("mg", "pcg"), default="mg",
+                    help="inner solver of the modal run: multigrid flexible CG or block-Jacobi block PCG")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (the known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -160,10 +162,11 @@ def time_solves(mesh_spec, all_forms=False):
     return out
 
 
-def time_modal(spec):
-    """Shift-invert block Lanczos on a twisted blade: wall time, inner block
-    PCG iterations, and the shift-invert residual lam*||K^-1 M x - x/lam||/||x||
-    of each returned mode (K^-1 by a tight FP64 block PCG)."""
+def time_modal(spec, inner="mg"):
+    """Shift-invert block Lanczos on a twisted blade: wall time, inner-solve
+    iterations per Lanczos step, and the shift-invert residual
+    lam*||K^-1 M x - x/lam||/||x|| of the first returned modes (K^-1 by a
+    tight solve of the same inner solver)."""
     import torch
     from fem_taichi_amd import Model, Physics, mesh
     from fem_taichi_amd.solve_eigen import Eigen_solver
@@ -173,10 +176,14 @@ def time_modal(spec):
     g = mesh.twisted_blade(nx, ny, nz)
     phys = Physics.from_grid(g)
     model = Model(g.name, phys, g.Materials, Omegas=(0, 0, 0))
+    from fem_taichi_amd import multigrid
     solver = Eigen_solver(model, 0)
+    th = time.perf_counter()
+    multigrid.hierarchy_for(model)          # symbolic, once per mesh (as the fine pattern)
+    hier_s = time.perf_counter() - th
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10)
+    freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10, inner=inner)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     lam = solver.eigenvalues
@@ -189,7 +196,7 @@ def time_modal(spec):
         lc = float(lam[c])
         res.append(float(lc * torch.linalg.vector_norm(Y[:, c] - Xd[:, c] / lc) / torch.linalg.vector_norm(Xd[:, c])))
     out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "modes": modes, "block": block,
-           "seconds": secs, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
+           "inner": inner, "seconds": secs, "hierarchy_seconds": hier_s, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
            "shift_invert_residual_first8": res}
     print(f"# modal {json.dumps(out)}", flush=True)
     return out
@@ -271,7 +278,7 @@ def main():
     if not args.no_solve:
         result["static_solve"] = time_solves(args.solve_mesh, all_forms=args.solve_all)
     if args.modal:
-        result["modal"] = time_modal(args.modal)
+        result["modal"] = time_modal(args.modal, args.modal_inner)
     if args.copy_probe:
         copy_probe()
     print(json.dumps(result), flush=True)

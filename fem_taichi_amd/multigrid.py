@@ -24,12 +24,13 @@ degree-1 Chebyshev (damped block-Jacobi) smoother for the interval
 V-cycle (FP32, all on the GPU): per level x = omega Dinv b, r = b - K x,
 b_c = P^T r, recurse, x += P x_c, x <- x + omega Dinv (b - K x); the
 coarsest level applies its dense inverse.  The outer iteration is FP64
-flexible CG (FCG(1)): its stopping test looks at the residual of the free
-DOFs, because the penalised rows (diagonal 1e20) carry almost no energy and a
-CG iteration reduces their residual only by rounding luck; once the free part
-has converged, one block-Jacobi step on the penalised rows (exact to
-rounding, since their diagonal dominates) removes theirs, and the full FP64
-residual is checked before returning.
+flexible CG (FCG(1)) on the free DOFs: the penalised rows (diagonal 1e20)
+carry almost no energy, so a Krylov iteration that includes them reduces
+their residual only by rounding luck and, once its residual is replaced by
+the true one, stalls on them; they are kept out of the iteration and set at
+the end by one block-Jacobi step on their own rows (exact to rounding, since
+their diagonal dominates), and the full FP64 residual is checked before
+returning.
 """
 from __future__ import annotations
 
@@ -316,36 +317,51 @@ class MultigridPreconditioner:
         return self.vcycle(r.float().contiguous()).double()
 
 
-def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, drop=1e-2):
-    """FP64 flexible CG (FCG(1)) with a nonlinear or inexact preconditioner.
+def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, drop=1e-2, free=None,
+                 dinv64=None, patience=3, restart=False):
+    """FP64 flexible CG (FCG(1)) with a nonlinear or inexact preconditioner,
+    run on the free DOFs only.
 
-    Stops when the free-DOF residual is below tol ||F|| and, after one
-    block-Jacobi correction on the penalised rows, the full FP64 residual is
-    too; replaces the recursive residual by F - K x whenever it has dropped by
-    ``drop``.  Returns (x, info dict)."""
+    The penalised rows (diagonal 1e20) are kept out of the Krylov iteration:
+    residuals, preconditioned residuals and K p are masked to the free DOFs,
+    so the iteration solves K_ff x_f = F_f (the coupling K_fp x_p is ~1e-15
+    relative, x_p being ~F / 1e20).  When the free residual is below
+    tol ||F|| (checked against the true F - K x), the penalised DOFs are set
+    by one block-Jacobi step on their rows, which is exact to rounding since
+    their diagonal dominates, and the full FP64 residual is computed.  The
+    recursive residual is replaced by the true one whenever it has dropped by
+    ``drop`` (and the search direction restarts when ``restart``).  FP64
+    cannot push the true residual below about kappa(K) * 1e-16: when
+    ``patience`` replacements in a row fail to halve the best true residual,
+    the iteration stops with ``stagnated`` set (the attainable accuracy, not
+    a divergence).  ``free`` (the free-DOF mask) and ``dinv64`` (K64's
+    block-Jacobi inverse) may be passed in when many systems share K.
+    Returns (x, info dict)."""
     from .learned import free_dof_mask
-    free = free_dof_mask(model)
+    if free is None:
+        free = free_dof_mask(model)
     fixed = 1.0 - free
-    dinv64, _ = K64.block_jacobi()
+    if dinv64 is None:
+        dinv64, _ = K64.block_jacobi()
     d3 = dinv64.view(-1, 3, 3)
     bn = float(torch.linalg.vector_norm(F))
     x = torch.zeros_like(F)
-    r = F.clone()
     if bn == 0.0:
-        return x, {"iterations": 0, "rel_residual": 0.0, "converged": True}
+        return x, {"iterations": 0, "rel_residual": 0.0, "converged": True, "trace": []}
+    r = F * free
     p_prev = q_prev = pq_prev = None
-    anchor = bn
+    anchor = float(torch.linalg.vector_norm(r))
     it = 0
-    rel = 1.0
-    converged = False
+    converged = stagnated = False
+    best_true, stalls = math.inf, 0
     trace = []
     while it < maxiter:
-        z = precond(r)
+        z = precond(r) * free
         if p_prev is None:
             p = z
         else:
             p = z - (torch.dot(z, q_prev) / pq_prev) * p_prev
-        q = K64.matvec(p)
+        q = K64.matvec(p) * free
         pq = torch.dot(p, q)
         alpha = torch.dot(p, r) / pq
         x.add_(alpha * p)
@@ -354,23 +370,32 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
         it += 1
         if it % check_every:
             continue
-        rf = float(torch.linalg.vector_norm(r * free))
+        rf = float(torch.linalg.vector_norm(r))
         if not math.isfinite(rf):
             break
         trace.append((it, rf / bn))
         if rf < drop * anchor or rf < tol * bn:
-            r = F - K64.matvec(x)
-            anchor = float(torch.linalg.vector_norm(r * free))
+            r = (F - K64.matvec(x)) * free
+            anchor = float(torch.linalg.vector_norm(r))
             if anchor < tol * bn:
-                x.add_(torch.bmm(d3, r.view(-1, 3, 1)).view(-1) * fixed)
-                r = F - K64.matvec(x)
-                rel = float(torch.linalg.vector_norm(r)) / bn
-                if rel < tol:
-                    converged = True
+                converged = True
+                break
+            if anchor < 0.5 * best_true:
+                best_true, stalls = anchor, 0
+            else:
+                stalls += 1
+                if stalls >= patience:
+                    stagnated = True
                     break
-    if not converged:
-        rel = float(torch.linalg.vector_norm(F - K64.matvec(x))) / bn
-    return x, {"iterations": it, "rel_residual": rel, "converged": converged, "trace": trace}
+            if restart:
+                p_prev = None
+    # the penalised DOFs from their own rows
+    rp = (F - K64.matvec(x)) * fixed
+    x.add_(torch.bmm(d3, rp.view(-1, 3, 1)).view(-1) * fixed)
+    rel = float(torch.linalg.vector_norm(F - K64.matvec(x))) / bn
+    converged = converged and rel < tol
+    return x, {"iterations": it, "rel_residual": rel, "converged": converged, "stagnated": stagnated,
+               "trace": trace}
 
 
 def solve_mg(model, K64, F, tol=1e-8, maxiter=1000, min_nodes=1000, fallback=True, **kw):

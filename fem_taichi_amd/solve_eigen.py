@@ -10,8 +10,9 @@ block Lanczos in the M inner product with an upper-Cholesky M-orthonormal
 start block, two full reorthogonalisation passes per step, Rayleigh-Ritz on
 the block tridiagonal T, lambda = 1/theta, ascending order, zero-padded T
 when num_block does not divide 2*num_eigen.  Differences, each stated:
-the inner solves K^-1 (M u) are block-Jacobi PCG on the GPU (to ``inner_tol``)
-instead of a Pardiso factorisation; the start block is drawn from
+the inner solves K^-1 (M u) are iterative GPU solves to ``inner_tol``
+instead of a Pardiso factorisation (FP64 flexible CG with the FP32 multigrid
+V-cycle on structured grids, multi-RHS block-Jacobi PCG otherwise); the start block is drawn from
 ``np.random.default_rng(seed)`` (the reference's is unseeded); the dense
 b x b / m x m work (Cholesky, inverse, eigh of T) and the tall-skinny
 reorthogonalisation products run as FP64 library calls on the device.
@@ -105,6 +106,45 @@ class BlockOps:
         return X[:, :b].contiguous()
 
 
+class MultigridBlockOps:
+    """K (the model's FP64 BSR) and an M-apply, with the inner solves
+    K^-1 B done column by column by FP64 flexible CG preconditioned by the
+    FP32 multigrid V-cycle (structured node grids; multigrid.py).  One
+    V-cycle set-up serves every solve of the eigen-solve.  A column stops at
+    ``tol`` or at the attainable FP64 accuracy (its true residual no longer
+    improving); one that fails otherwise within ``maxiter`` is finished by
+    FP64 block-Jacobi PCG."""
+
+    def __init__(self, model, m_apply, min_nodes=1000):
+        from . import multigrid, solvers
+        from .learned import free_dof_mask
+        self.model = model
+        self.K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+        self.pre = multigrid.MultigridPreconditioner(model, multigrid.hierarchy_for(model, min_nodes))
+        self.dinv64, _ = self.K64.block_jacobi()
+        self.free = free_dof_mask(model)
+        self.m_apply = m_apply
+        self.n = 3 * model.Num_Node
+        self.device = device()
+        self.iterations = []
+
+    def solve(self, B, tol, maxiter=1000):
+        from . import multigrid, solvers
+        X = torch.empty_like(B)
+        total = 0
+        for j in range(B.shape[1]):
+            b = B[:, j].contiguous()
+            x, info = multigrid.solve_fcg_mg(self.model, self.K64, b, self.pre, tol=tol, maxiter=maxiter,
+                                             free=self.free, dinv64=self.dinv64)
+            total += info["iterations"]
+            if not (info["converged"] or info["stagnated"]):
+                x, it2, _ = solvers.pcg(self.K64, b, self.dinv64, tol, 200000, x0=x.clone())
+                total += it2
+            X[:, j] = x
+        self.iterations.append(total)
+        return X
+
+
 def _chol_upper(A):
     """scipy.linalg.cholesky(A) (upper, A = U^T U)."""
     return torch.linalg.cholesky(A).transpose(0, 1).contiguous()
@@ -164,11 +204,18 @@ class Eigen_solver:
         self.Model = Model
         self.ops = None
 
-    def Solve_cpu(self, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, verbose=False):
+    def Solve_cpu(self, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, verbose=False, inner="auto"):
+        """``inner``: "mg" (multigrid flexible CG, structured grids), "pcg"
+        (multi-RHS block-Jacobi PCG) or "auto" (mg when the Physics has
+        node_dims)."""
         Mo = self.Model
         kv = Mo.K_bsr("f64")
         mb = Mo.M_blocks()
         dev_mesh = Mo.dev
+        if inner == "auto":
+            inner = "mg" if getattr(Mo.Physics, "node_dims", None) is not None else "pcg"
+        if inner not in ("mg", "pcg"):
+            raise ValueError(f"unknown inner solver {inner!r}")
 
         def m_apply(X):
             X = X.contiguous()
@@ -177,7 +224,7 @@ class Eigen_solver:
                       stream_ptr())
             return Y
 
-        self.ops = BlockOps(dev_mesh, kv, m_apply)
+        self.ops = MultigridBlockOps(Mo, m_apply) if inner == "mg" else BlockOps(dev_mesh, kv, m_apply)
         lam, X, T = lanczos(self.ops, num_eigen, num_block, seed=seed, inner_tol=inner_tol)
         self.eigenvalues = lam
         freq = torch.sqrt(lam) / 2 / np.pi

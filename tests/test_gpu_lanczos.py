@@ -103,3 +103,30 @@ def test_eigen_solver_ansys_mesh2(gpu):
     lam = solver.eigenvalues.cpu().numpy()
     print("lambda", lam[:6], "eigsh", ref, flush=True)
     assert np.allclose(lam[:5], ref[:5], rtol=EIG_TOL, atol=0)
+
+
+def test_eigen_solver_multigrid_inner_matches_pcg_inner(gpu):
+    """Inner solves by multigrid flexible CG (a 2-level hierarchy on a twisted
+    blade) against the same Lanczos with block-Jacobi PCG inner solves and
+    against eigsh on the oracle's K and M: the lowest eigenvalues to 1e-8."""
+    from fem_taichi_amd import Model, Physics
+    from fem_taichi_amd.solve_eigen import Eigen_solver
+    g = mesh.twisted_blade(40, 6, 6)
+    print("\ncase blade 40x6x6 modes, inner mg vs pcg", flush=True)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=(0, 0, 0))
+    pp = O.pre_process(g.Node, g.Elem)
+    K = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
+    M = O.model_M_csr(g.Node, g.Elem, pp, g.DE)
+    ref = np.sort(sla.eigsh(K.tocsc(), k=4, M=M.tocsr(), sigma=0, which='LM', return_eigenvectors=False))
+    s_mg = Eigen_solver(model, 0)
+    s_mg.Solve_cpu(num_eigen=16, num_block=4, seed=5, inner="mg")
+    assert len(s_mg.ops.pre.hier.dims) >= 2
+    s_pcg = Eigen_solver(model, 0)
+    s_pcg.Solve_cpu(num_eigen=16, num_block=4, seed=5, inner="pcg")
+    lam_mg = s_mg.eigenvalues.cpu().numpy()
+    lam_pcg = s_pcg.eigenvalues.cpu().numpy()
+    print("mg", lam_mg[:4], "pcg", lam_pcg[:4], "eigsh", ref, "inner its mg", s_mg.ops.iterations[:3],
+          "pcg", s_pcg.ops.iterations[:3], flush=True)
+    assert np.allclose(lam_mg[:3], ref[:3], rtol=EIG_TOL, atol=0)
+    assert np.allclose(lam_mg[:3], lam_pcg[:3], rtol=EIG_TOL, atol=0)
