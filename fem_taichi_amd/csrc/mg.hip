@@ -8,33 +8,39 @@
 //   are gathers (thread per output node), so no atomics and a fixed
 //   summation order.
 // * The damped block-Jacobi smoother x <- x + w Dinv (b - K x) and the
-//   residual r = b - K x on the BSR(3x3) FP32 matrix: 32 lanes per block row
-//   (lane = block of the row, so the row's contiguous run of blocks is read
-//   coalesced), the three row sums reduced with lane shuffles.
+//   residual r = b - K x on the FP32 matrix in BSR or the reference's CSR
+//   value order: 16 lanes per block row (lane b takes blocks b, b + 16, so
+//   the row's contiguous run of values is read coalesced), the three row
+//   sums reduced with lane shuffles.
 // * y = A x for the small dense inverse of the coarsest level.
 #include "common.hpp"
 
 namespace fem {
 namespace {
 
-constexpr int kLanes = 32;
+constexpr int kLanes = 16;      // lanes per block row (FP32 rows: 16 beats 32 and 8, measured)
 constexpr int kThreads = 256;
 constexpr int kRowsPerBlock = kThreads / kLanes;
 
-// (K x)_i for block row i over the row's blocks; lanes b, b + 32, ...
+// (K x)_i for block row i over the row's blocks; lanes b, b + 16, ...  Entry
+// (p, q) of the row's block b sits at 9 (sm + b) + 3p + q (BSR) or at
+// 9 sm + 3p co + 3b + q (the reference's CSR order).
+template <int LAYOUT>
 __device__ __forceinline__ void row_mul(const fem_mesh& m, const float* __restrict__ vals,
                                         const float* __restrict__ x, int64_t i, int lane, float& a0,
                                         float& a1, float& a2) {
   a0 = a1 = a2 = 0.0f;
   const int64_t sm = m.sm_node[i];
   const int co = m.co_node[i];
+  const int64_t ps = LAYOUT == FEM_LAYOUT_CSR ? 3 * (int64_t)co : 3;    // stride between block rows p
+#pragma unroll 2
   for (int b = lane; b < co; b += kLanes) {
-    const float* __restrict__ v = vals + 9 * (sm + b);
+    const float* __restrict__ v = LAYOUT == FEM_LAYOUT_CSR ? vals + 9 * sm + 3 * b : vals + 9 * (sm + b);
     const int64_t c = 3 * (int64_t)m.re_node[sm + b];
     const float x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
     a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
-    a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
-    a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    a1 += v[ps] * x0 + v[ps + 1] * x1 + v[ps + 2] * x2;
+    a2 += v[2 * ps] * x0 + v[2 * ps + 1] * x1 + v[2 * ps + 2] * x2;
   }
 #pragma unroll
   for (int o = kLanes / 2; o > 0; o >>= 1) {
@@ -45,16 +51,16 @@ __device__ __forceinline__ void row_mul(const fem_mesh& m, const float* __restri
 }
 
 // MODE 0: out = b - K x.  MODE 1: out = x + w Dinv (b - K x) (out != x).
-template <int MODE>
+template <int MODE, int LAYOUT>
 __global__ void __launch_bounds__(kThreads)
 smooth_kernel(fem_mesh m, const float* __restrict__ vals, const float* __restrict__ dinv, float w,
               const float* __restrict__ x, const float* __restrict__ b, float* __restrict__ out) {
   const int lane = threadIdx.x & (kLanes - 1);
   const int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kLanes;
-  // the 32 lanes of a row take the same branch, so the shuffles see whole groups
+  // the lanes of a row take the same branch, so the shuffles see whole groups
   if (i < m.num_node) {
     float a0, a1, a2;
-    row_mul(m, vals, x, i, lane, a0, a1, a2);
+    row_mul<LAYOUT>(m, vals, x, i, lane, a0, a1, a2);
     if (lane == 0) {
       const float r0 = b[3 * i] - a0, r1 = b[3 * i + 1] - a1, r2 = b[3 * i + 2] - a2;
       if (MODE == 0) {
@@ -194,10 +200,11 @@ int check_transfer(const fem_grid_transfer* t) {
 
 using namespace fem;
 
-extern "C" int fem_mg_smooth(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+extern "C" int fem_mg_smooth(const fem_mesh* mesh, const float* vals, int layout, const float* dinv, float omega,
                              const float* x, const float* b, float* out, int mode, void* stream) {
   FEM_REQUIRE(mesh && b && out, FEM_E_INVALID, "null pointer");
   FEM_REQUIRE(mode >= 0 && mode <= 2, FEM_E_INVALID, "mode must be 0 (residual), 1 (update) or 2 (start)");
+  FEM_REQUIRE(layout == FEM_LAYOUT_BSR || layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad layout %d", layout);
   FEM_REQUIRE(mode == 0 || dinv, FEM_E_INVALID, "null dinv");
   FEM_REQUIRE(mode == 2 || (vals && x), FEM_E_INVALID, "null vals or x");
   FEM_REQUIRE(mode != 1 || out != x, FEM_E_INVALID, "the update needs out != x (x is gathered)");
@@ -206,10 +213,16 @@ extern "C" int fem_mg_smooth(const fem_mesh* mesh, const float* vals, const floa
   hipStream_t st = as_stream(stream);
   if (mode == 2) {
     jacobi_start_kernel<<<ceil_div(N, kThreads), kThreads, 0, st>>>(N, dinv, omega, b, out);
-  } else if (mode == 0) {
-    smooth_kernel<0><<<ceil_div(N, kRowsPerBlock), kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
   } else {
-    smooth_kernel<1><<<ceil_div(N, kRowsPerBlock), kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+    const int64_t nb = ceil_div(N, kRowsPerBlock);
+    if (mode == 0 && layout == FEM_LAYOUT_BSR)
+      smooth_kernel<0, FEM_LAYOUT_BSR><<<nb, kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+    else if (mode == 0)
+      smooth_kernel<0, FEM_LAYOUT_CSR><<<nb, kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+    else if (layout == FEM_LAYOUT_BSR)
+      smooth_kernel<1, FEM_LAYOUT_BSR><<<nb, kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
+    else
+      smooth_kernel<1, FEM_LAYOUT_CSR><<<nb, kThreads, 0, st>>>(*mesh, vals, dinv, omega, x, b, out);
   }
   FEM_LAUNCH_CHECK();
   return FEM_OK;

@@ -39,6 +39,20 @@ __device__ double sum_partials(const double* __restrict__ part, double* red) {
   return block_sum(s, red);
 }
 
+// dst[b] = sum of src[b*chunk .. (b+1)*chunk) for the kGrid blocks b, chunk =
+// ceil(n / kGrid) (fixed order: deterministic).
+__global__ void __launch_bounds__(kThreads)
+fold_partials_kernel(const double* __restrict__ src, int64_t n, double* __restrict__ dst) {
+  __shared__ double red[kThreads / 64];
+  const int64_t chunk = (n + kGrid - 1) / kGrid;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  double s = 0.0;
+  for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) s += src[j];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) dst[blockIdx.x] = s;
+}
+
 template <typename VT, typename XT>
 __device__ __forceinline__ void row_product(const fem_mesh& m, const VT* __restrict__ vals,
                                             const XT* __restrict__ x, int64_t i, int lane,
@@ -46,6 +60,7 @@ __device__ __forceinline__ void row_product(const fem_mesh& m, const VT* __restr
   a0 = a1 = a2 = XT(0);
   const int64_t sm = m.sm_node[i];
   const int co = m.co_node[i];
+#pragma unroll 2
   for (int b = lane; b < co; b += kRowLanes) {
     const VT* __restrict__ v = vals + 9 * (sm + b);
     const int64_t c = 3 * (int64_t)m.re_node[sm + b];
@@ -62,19 +77,20 @@ __device__ __forceinline__ void row_product(const fem_mesh& m, const VT* __restr
   }
 }
 
-// y = A x (grid-stride over rows).  With DOT, partial x.y per block into
+// y = A x (one row per 32-lane team).  With DOT, partial x.y per block into
 // part[blockIdx.x]; block 0 also moves state[1] (rz of the previous
 // iteration) into state[0] for the update kernel that follows.
-template <typename VT, typename XT, bool DOT>
-__global__ void __launch_bounds__(kThreads)
+template <typename VT, typename XT, bool DOT, int NT = kThreads>
+__global__ void __launch_bounds__(NT)
 spmv_kernel(fem_mesh m, const VT* __restrict__ vals, const XT* __restrict__ x, XT* __restrict__ y,
             double* __restrict__ part, double* state) {
-  __shared__ double red[kThreads / 64];
+  __shared__ double red[NT / 64];
+  constexpr int kRows = NT / kRowLanes;
   const int lane = threadIdx.x & (kRowLanes - 1);
   double d = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * kRowsPerBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kRowsPerBlock + threadIdx.x / kRowLanes; i < m.num_node;
-       i += stride) {
+  // one row per lane team: the grid covers every row (the launch sites use ceil(N / kRows) blocks)
+  const int64_t i = (int64_t)blockIdx.x * kRows + threadIdx.x / kRowLanes;
+  if (i < m.num_node) {
     XT a0, a1, a2;
     row_product(m, vals, x, i, lane, a0, a1, a2);
     if (lane == 0) {
@@ -332,8 +348,8 @@ int64_t grid_for(int64_t items, int per_block) {
 using namespace fem;
 
 extern "C" int64_t fem_pcg_work_bytes(int64_t num_node) {
-  (void)num_node;
-  return 8 * (int64_t)kGrid * 6 + 256;
+  // six kGrid-sized partial areas + one partial per SpMV work-group
+  return 8 * ((int64_t)kGrid * 6 + ceil_div(num_node, kRowsPerBlock)) + 256;
 }
 
 extern "C" int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, const void* x, void* y,
@@ -420,7 +436,12 @@ template <typename VT, typename XT>
 static int pcg_iterate_t(const fem_mesh* mesh, const VT* vals, const VT* dinv, XT* x, XT* r, XT* z, XT* p,
                          XT* q, double* state, double* part, hipStream_t st) {
   const int64_t N = mesh->num_node;
-  spmv_kernel<VT, XT, true><<<kGrid, kThreads, 0, st>>>(*mesh, vals, p, q, part, state);
+  // the SpMV runs on the full row grid (one partial p.q per work-group, into the
+  // scratch after the kGrid-sized areas); the fold sums them, in order, into kGrid partials
+  const int64_t nblk = ceil_div(N, kRowsPerBlock);
+  double* rows_part = part + 6 * kGrid;
+  spmv_kernel<VT, XT, true><<<nblk, kThreads, 0, st>>>(*mesh, vals, p, q, rows_part, state);
+  fold_partials_kernel<<<kGrid, kThreads, 0, st>>>(rows_part, nblk, part);
   pcg_update_kernel<VT, XT><<<kGrid, kThreads, 0, st>>>(N, x, r, z, p, q, dinv, part,
                                                                        part + kGrid, state);
   pcg_direction_kernel<XT><<<grid_for(3 * N, kThreads), kThreads, 0, st>>>(3 * N, z, p, part + kGrid, state);

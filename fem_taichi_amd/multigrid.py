@@ -228,8 +228,7 @@ class MultigridPreconditioner:
             self.meshes.append(m.dev)
             if l < L - 1:
                 k32 = m.K_bsr("f32") if l == 0 else m.assemble_K("f32", "bsr")
-                op = BsrOperator(m.dev, k32)
-                dinv, _ = op.block_jacobi()
+                dinv, _ = BsrOperator(m.dev, k32).block_jacobi()
                 self.k32.append(k32)
                 self.dinv.append(dinv)
                 self.omega.append(1.0 / self._theta(m.dev, k32, dinv, seed + l))
@@ -257,10 +256,10 @@ class MultigridPreconditioner:
         y = torch.empty_like(x)
         lam = torch.zeros((), dtype=torch.float32, device=x.device)
         for _ in range(EIG_ITERS):
-            _lib.call("fem_mg_smooth", dev.ref, k32.data_ptr(), dinv.data_ptr(), 1.0, x.data_ptr(), zero.data_ptr(),
-                      r.data_ptr(), 0, st)                       # r = -K x
-            _lib.call("fem_mg_smooth", dev.ref, None, dinv.data_ptr(), -1.0, None, r.data_ptr(), y.data_ptr(), 2,
-                      st)                                        # y = Dinv K x
+            _lib.call("fem_mg_smooth", dev.ref, k32.data_ptr(), _lib.LAYOUT_BSR, dinv.data_ptr(), 1.0, x.data_ptr(),
+                      zero.data_ptr(), r.data_ptr(), 0, st)      # r = -K x
+            _lib.call("fem_mg_smooth", dev.ref, None, _lib.LAYOUT_BSR, dinv.data_ptr(), -1.0, None, r.data_ptr(),
+                      y.data_ptr(), 2, st)                       # y = Dinv K x
             lam = torch.linalg.vector_norm(y)
             x = y / lam
         lmax = float(lam)
@@ -295,8 +294,9 @@ class MultigridPreconditioner:
         for l in range(L - 1):
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
-            _lib.call("fem_mg_smooth", dev.ref, None, dv, w, None, self.b[l].data_ptr(), self.x[l].data_ptr(), 2, st)
-            _lib.call("fem_mg_smooth", dev.ref, k, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
+            _lib.call("fem_mg_smooth", dev.ref, None, _lib.LAYOUT_BSR, dv, w, None, self.b[l].data_ptr(),
+                      self.x[l].data_ptr(), 2, st)
+            _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
                       self.t[l].data_ptr(), 0, st)
             _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, self.t[l].data_ptr(), self.b[l + 1].data_ptr(), st)
         n = self.b[L - 1].numel()
@@ -307,7 +307,7 @@ class MultigridPreconditioner:
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, self.x[l + 1].data_ptr(), self.x[l].data_ptr(),
                       st)
-            _lib.call("fem_mg_smooth", dev.ref, k, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
+            _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
                       self.t[l].data_ptr(), 1, st)
             self.x[l], self.t[l] = self.t[l], self.x[l]
         return self.x[0]

@@ -264,10 +264,11 @@ typedef struct fem_grid_transfer {
   const int32_t* span;    /* 2 * (nc[0] + nc[1] + nc[2]) */
 } fem_grid_transfer;
 
-/* FP32 block-Jacobi smoother pieces on the BSR(3x3) FP32 matrix vals:
+/* FP32 block-Jacobi smoother pieces on the FP32 matrix vals, in layout
+ * FEM_LAYOUT_BSR or FEM_LAYOUT_CSR (the value orders of fem_hex8_assemble):
  * mode 0: out = b - K x;  mode 1: out = x + omega Dinv (b - K x) (out must
  * not alias x);  mode 2: out = omega Dinv b (vals and x unused). */
-int fem_mg_smooth(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+int fem_mg_smooth(const fem_mesh* mesh, const float* vals, int layout, const float* dinv, float omega,
                   const float* x, const float* b, float* out, int mode, void* stream);
 /* coarse = P^T fine (3 FP32 values per node). */
 int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, void* stream);

@@ -43,7 +43,8 @@ def test_transfers_match_interpolation_matrix(gpu, dims):
     assert rel(y.cpu().numpy(), xf + P @ xc.astype(np.float64)) < 1e-6
 
 
-def test_smoother_modes(gpu):
+@pytest.mark.parametrize("layout", ["bsr", "csr"])
+def test_smoother_modes(gpu, layout):
     import torch
     from fem_taichi_amd import Model, Physics, _lib, solvers
     from fem_taichi_amd.runtime import stream_ptr
@@ -52,8 +53,9 @@ def test_smoother_modes(gpu):
     model = Model(g.name, phys, g.Materials, Omegas=g.Omegas)
     pp = O.pre_process(g.Node, g.Elem)
     K = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
-    k32 = model.K_bsr("f32")
-    dinv, _ = solvers.BsrOperator(model.dev, k32).block_jacobi()
+    k32 = model.assemble_K("f32", layout)
+    lay = _lib.LAYOUT_BSR if layout == "bsr" else _lib.LAYOUT_CSR
+    dinv, _ = solvers.BsrOperator(model.dev, model.K_bsr("f32")).block_jacobi()
     n = K.shape[0]
     rng = np.random.default_rng(2)
     x = rng.standard_normal(n).astype(np.float32)
@@ -65,15 +67,16 @@ def test_smoother_modes(gpu):
     out = torch.empty_like(xd)
     st = stream_ptr()
     r_ref = b - K @ x.astype(np.float64)
-    _lib.call("fem_mg_smooth", model.dev.ref, k32.data_ptr(), dinv.data_ptr(), w, xd.data_ptr(), bd.data_ptr(),
+    _lib.call("fem_mg_smooth", model.dev.ref, k32.data_ptr(), lay, dinv.data_ptr(), w, xd.data_ptr(), bd.data_ptr(),
               out.data_ptr(), 0, st)
     free = np.abs(r_ref) < 1e15        # penalised rows: 1e20 x in FP32 and FP64 differ in the last bits
     assert rel(out.cpu().numpy()[free], r_ref[free]) < 1e-5
-    _lib.call("fem_mg_smooth", model.dev.ref, k32.data_ptr(), dinv.data_ptr(), w, xd.data_ptr(), bd.data_ptr(),
+    _lib.call("fem_mg_smooth", model.dev.ref, k32.data_ptr(), lay, dinv.data_ptr(), w, xd.data_ptr(), bd.data_ptr(),
               out.data_ptr(), 1, st)
     ref = x + w * (Dinv @ r_ref)
     assert rel(out.cpu().numpy(), ref) < 1e-5
-    _lib.call("fem_mg_smooth", model.dev.ref, None, dinv.data_ptr(), w, None, bd.data_ptr(), out.data_ptr(), 2, st)
+    _lib.call("fem_mg_smooth", model.dev.ref, None, lay, dinv.data_ptr(), w, None, bd.data_ptr(), out.data_ptr(), 2,
+              st)
     assert rel(out.cpu().numpy(), w * (Dinv @ b.astype(np.float64))) < 1e-5
 
 
