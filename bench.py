@@ -53,6 +53,8 @@ def parse():
                     help="also time block Lanczos (Eigen_solver) on a twisted blade, e.g. 400x50x50:20:8")
     ap.add_argument("--modal-inner", choices=("mg", "pcg"), default="mg",
                     help="inner solver of the modal run: multigrid flexible CG or block-Jacobi block PCG")
+    ap.add_argument("--modal-prestress", action="store_true",
+                    help="modal run on the spinning blade: stress stiffening + spin softening")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a plain 256 MiB device copy (the known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -162,11 +164,14 @@ def time_solves(mesh_spec, all_forms=False):
     return out
 
 
-def time_modal(spec, inner="mg"):
+def time_modal(spec, inner="mg", prestress=False):
     """Shift-invert block Lanczos on a twisted blade: wall time, inner-solve
     iterations per Lanczos step, and the shift-invert residual
     lam*||K^-1 M x - x/lam||/||x|| of the first returned modes (K^-1 by a
-    tight solve of the same inner solver)."""
+    tight solve of the same inner solver).  With ``prestress`` the blade
+    spins at its omega = (0, 0, 100) rad/s: stress stiffening from its own
+    centrifugal static solution plus spin softening (the static solve is
+    inside the timed region)."""
     import torch
     from fem_taichi_amd import Model, Physics, mesh
     from fem_taichi_amd.solve_eigen import Eigen_solver
@@ -175,7 +180,7 @@ def time_modal(spec, inner="mg"):
     modes, block = int(modes), int(block)
     g = mesh.twisted_blade(nx, ny, nz)
     phys = Physics.from_grid(g)
-    model = Model(g.name, phys, g.Materials, Omegas=(0, 0, 0))
+    model = Model(g.name, phys, g.Materials, Omegas=g.Omegas if prestress else (0, 0, 0))
     from fem_taichi_amd import multigrid
     solver = Eigen_solver(model, 0)
     th = time.perf_counter()
@@ -183,7 +188,8 @@ def time_modal(spec, inner="mg"):
     hier_s = time.perf_counter() - th
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10, inner=inner)
+    freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10, inner=inner,
+                               prestress="static" if prestress else None)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t0
     lam = solver.eigenvalues
@@ -196,7 +202,7 @@ def time_modal(spec, inner="mg"):
         lc = float(lam[c])
         res.append(float(lc * torch.linalg.vector_norm(Y[:, c] - Xd[:, c] / lc) / torch.linalg.vector_norm(Xd[:, c])))
     out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "modes": modes, "block": block,
-           "inner": inner, "seconds": secs, "hierarchy_seconds": hier_s, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
+           "inner": inner, "operator": solver.operator, "seconds": secs, "hierarchy_seconds": hier_s, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
            "shift_invert_residual_first8": res}
     print(f"# modal {json.dumps(out)}", flush=True)
     return out
@@ -278,7 +284,7 @@ def main():
     if not args.no_solve:
         result["static_solve"] = time_solves(args.solve_mesh, all_forms=args.solve_all)
     if args.modal:
-        result["modal"] = time_modal(args.modal, args.modal_inner)
+        result["modal"] = time_modal(args.modal, args.modal_inner, args.modal_prestress)
     if args.copy_probe:
         copy_probe()
     print(json.dumps(result), flush=True)

@@ -83,6 +83,8 @@ SIGNATURES = {
     "fem_hex8_assemble_lds_bytes": (_I64, [ctypes.c_int32] * 2 + [_I32, _I32, _I32]),
     "fem_hex8_assemble_kn": (_I32, [_MP, _P, _D, _D, _P, _P, _P, _I32, _P]),
     "fem_hex8_kn_lds_bytes": (_I64, [ctypes.c_int32, ctypes.c_int32]),
+    "fem_hex8_assemble_ks": (_I32, [_MP, _P, _D, _D, _P, _P, _I32, _P]),
+    "fem_spin_softening": (_I32, [_MP, _P, ctypes.POINTER(ctypes.c_double), _P, _I32, _P]),
     "fem_apply_penalty": (_I32, [_MP, _P, _P, _I64, _P, _I32, _I32, _D, _P]),
     "fem_body_force_work_bytes": (_I64, [_I64]),
     "fem_body_force": (_I32, [_MP, _D, ctypes.POINTER(ctypes.c_double), _P, _P, _P, _I64, _P]),

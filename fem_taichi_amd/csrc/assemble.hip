@@ -409,6 +409,31 @@ __global__ void csr_index_M_kernel(fem_mesh m, int32_t* __restrict__ row_m, int3
 }
 
 // ---------------------------------------------------------------------------
+// Spin softening (no counterpart in the reference; Readme.md:189 lists it as
+// missing): the centrifugal load rho (|w|^2 x - (w.x) w) grows with the
+// displacement, a negative stiffness K_jk -= m_jk (|w|^2 I - w w^T), m_jk the
+// consistent-mass scalar of the node pair.  Thread per block row.
+// ---------------------------------------------------------------------------
+template <int LAYOUT>
+__global__ void spin_kernel(fem_mesh m, const double* __restrict__ mb, double w0, double w1, double w2,
+                            double* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.num_node) return;
+  const int64_t sm = m.sm_node[i];
+  const int co = m.co_node[i];
+  const double w[3] = {w0, w1, w2};
+  const double ww = w0 * w0 + w1 * w1 + w2 * w2;
+  for (int k = 0; k < co; ++k) {
+    const double mk = mb[sm + k];
+#pragma unroll
+    for (int idx = 0; idx < 9; ++idx) {
+      const int p = idx / 3, q = idx % 3;
+      vals[k_addr<LAYOUT>(sm, co, k, idx)] -= mk * ((p == q ? ww : 0.0) - w[p] * w[q]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Penalty (FEM_model_csr.py:233-242 / FEM_model_bsr.py:248-259): SET diagonal.
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -445,6 +470,12 @@ __global__ void penalty_kernel(fem_mesh m, const int32_t* __restrict__ bnode,
 // + 2 mu E, E the Green strain, into one element field inside its Gauss
 // loop), taken as an inclusive scan over the element's 8 Gauss lanes.  Same
 // row groups / task lists / lane butterfly as the linear kernel; FP64 only.
+//
+// MODE 1 (stress stiffening of a linear prestress, no counterpart in the
+// reference, whose Readme.md:189 lists it as missing): F = I and s = the
+// Gauss point's own linear stress lambda tr(eps) I + 2 mu eps, eps =
+// sym(du/dx), so KT = K_L + K_sigma(u) with K_sigma_jk = sum_g detJ
+// (a_j^T s_g a_k) I; only KT is written.
 // ---------------------------------------------------------------------------
 constexpr int kKnThreads = 256;
 constexpr int kKnG = 24;   // per (e, g): wl, wm, F[9], FFt[6], st[6] (xx yy zz xy xz yz), pad
@@ -519,7 +550,7 @@ __device__ __forceinline__ void store_raw(double* __restrict__ vals, const int32
   for (int idx = 0; idx < 9; ++idx) vals[k_addr<LAYOUT>(sm, co, m, idx)] = v[idx];
 }
 
-template <int LAYOUT>
+template <int LAYOUT, int MODE>
 __global__ void __launch_bounds__(kKnThreads)
 hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar, double lhat, double mhat,
                const double* __restrict__ u, double* __restrict__ kt, double* __restrict__ km, int eg, int rg) {
@@ -593,15 +624,23 @@ hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar
 #pragma unroll
         for (int p = 0; p < 3; ++p) T[3 * r + p] = fma(uk[r], a[p], T[3 * r + p]);
     }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) F[q] = T[q] + ((q % 4 == 0) ? 1.0 : 0.0);
-    // Green strain E = (F^T F - I) / 2 and the stress lambda tr(E) I + 2 mu E
     double Eg[9];
+    if (MODE == 0) {
 #pragma unroll
-    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 9; ++q) F[q] = T[q] + ((q % 4 == 0) ? 1.0 : 0.0);
+      // Green strain E = (F^T F - I) / 2 and the stress lambda tr(E) I + 2 mu E
 #pragma unroll
-      for (int q = 0; q < 3; ++q)
-        Eg[3 * p + q] = 0.5 * (F[p] * F[q] + F[3 + p] * F[3 + q] + F[6 + p] * F[6 + q] - (p == q ? 1.0 : 0.0));
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          Eg[3 * p + q] = 0.5 * (F[p] * F[q] + F[3 + p] * F[3 + q] + F[6 + p] * F[6 + q] - (p == q ? 1.0 : 0.0));
+    } else {
+      // F stays I; the small strain sym(du/dx)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) Eg[3 * p + q] = 0.5 * (T[3 * p + q] + T[3 * q + p]);
+    }
     const double lam = Ee * lhat, mu = Ee * mhat;
     const double tr = Eg[0] + Eg[4] + Eg[8];
     S[0] = lam * tr + 2 * mu * Eg[0];
@@ -612,12 +651,14 @@ hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar
     S[5] = 2 * mu * Eg[5];
   }
   // inclusive scan of the stress over the element's Gauss points (8 lanes)
+  if (MODE == 0) {
 #pragma unroll
-  for (int off = 1; off < 8; off <<= 1) {
+    for (int off = 1; off < 8; off <<= 1) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      const double t = __shfl_up(S[c], off, 8);
-      if (g >= off) S[c] += t;
+      for (int c = 0; c < 6; ++c) {
+        const double t = __shfl_up(S[c], off, 8);
+        if (g >= off) S[c] += t;
+      }
     }
   }
   if (p1) {
@@ -656,7 +697,7 @@ hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar
     if (dual) {   // a dual task's first block is complete: store it, then reuse acc
       if (valid) {
         store_raw<LAYOUT>(kt, s_sm, s_co, outA, acc);
-        store_raw<LAYOUT>(km, s_sm, s_co, outA, acc + 9);
+        if (MODE == 0) store_raw<LAYOUT>(km, s_sm, s_co, outA, acc + 9);
       }
 #pragma unroll
       for (int q = 0; q < 18; ++q) acc[q] = 0.0;
@@ -665,7 +706,7 @@ hex8_kn_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar
     block_allsum(acc, Tn);
     if (valid && (lane & (Tn - 1)) == 0 && (!dual || has_b)) {
       store_raw<LAYOUT>(kt, s_sm, s_co, dual ? outB : outA, acc);
-      store_raw<LAYOUT>(km, s_sm, s_co, dual ? outB : outA, acc + 9);
+      if (MODE == 0) store_raw<LAYOUT>(km, s_sm, s_co, dual ? outB : outA, acc + 9);
     }
   }
 }
@@ -736,11 +777,11 @@ extern "C" int64_t fem_hex8_kn_lds_bytes(int32_t max_group_elems, int32_t max_gr
   return kn_lds_layout(max_group_elems, max_group_rows).total;
 }
 
-extern "C" int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
-                                    const double* u, double* kt_vals, double* km_vals, int k_layout, void* stream) {
+static int launch_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu, const double* u,
+                     double* kt_vals, double* km_vals, int k_layout, int mode, void* stream) {
   int rc = check_mesh(mesh, true);
   if (rc) return rc;
-  FEM_REQUIRE(u && kt_vals && km_vals, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(u && kt_vals && (mode == 1 || km_vals), FEM_E_INVALID, "null pointer");
   FEM_REQUIRE(k_layout == FEM_LAYOUT_BSR || k_layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad k_layout");
   FEM_REQUIRE(nu > -1.0 && nu < 0.5, FEM_E_INVALID, "Poisson ratio %g outside (-1, 0.5)", nu);
   FEM_REQUIRE(mesh->max_group_elems * 8 <= kKnThreads, FEM_E_LIMIT,
@@ -752,13 +793,24 @@ extern "C" int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, d
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
   hipStream_t st = as_stream(stream);
-  auto kern = k_layout == FEM_LAYOUT_BSR ? hex8_kn_kernel<0> : hex8_kn_kernel<1>;
+  auto kern = mode == 0 ? (k_layout == FEM_LAYOUT_BSR ? hex8_kn_kernel<0, 0> : hex8_kn_kernel<1, 0>)
+                        : (k_layout == FEM_LAYOUT_BSR ? hex8_kn_kernel<0, 1> : hex8_kn_kernel<1, 1>);
   FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
   hipLaunchKernelGGL(kern, dim3((unsigned)mesh->num_groups), dim3(kKnThreads), L.total, st, *mesh, young,
                      young_scalar, lhat, mhat, u, kt_vals, km_vals, eg, rg);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
+}
+
+extern "C" int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
+                                    const double* u, double* kt_vals, double* km_vals, int k_layout, void* stream) {
+  return launch_kn(mesh, young, young_scalar, nu, u, kt_vals, km_vals, k_layout, 0, stream);
+}
+
+extern "C" int fem_hex8_assemble_ks(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
+                                    const double* u, double* k_vals, int k_layout, void* stream) {
+  return launch_kn(mesh, young, young_scalar, nu, u, k_vals, nullptr, k_layout, 1, stream);
 }
 
 extern "C" int fem_csr_index_K(const fem_mesh* mesh, int32_t* row_k, int32_t* col_k, void* stream) {
@@ -798,6 +850,23 @@ extern "C" int fem_csr_values_M(const fem_mesh* mesh, const double* m_blocks, do
   FEM_REQUIRE(m_blocks && val_m, FEM_E_INVALID, "null pointer");
   csr_index_M_kernel<<<ceil_div(mesh->nnzb, 256), 256, 0, as_stream(stream)>>>(
       *mesh, nullptr, nullptr, m_blocks, val_m);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_spin_softening(const fem_mesh* mesh, const double* m_blocks, const double* omega3,
+                                  double* k_vals, int k_layout, void* stream) {
+  int rc = check_mesh(mesh, false);
+  if (rc) return rc;
+  FEM_REQUIRE(m_blocks && omega3 && k_vals, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(k_layout == FEM_LAYOUT_BSR || k_layout == FEM_LAYOUT_CSR, FEM_E_INVALID, "bad layout");
+  if (mesh->num_node == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  const int64_t nb = ceil_div(mesh->num_node, 256);
+  if (k_layout == FEM_LAYOUT_BSR)
+    spin_kernel<FEM_LAYOUT_BSR><<<nb, 256, 0, st>>>(*mesh, m_blocks, omega3[0], omega3[1], omega3[2], k_vals);
+  else
+    spin_kernel<FEM_LAYOUT_CSR><<<nb, 256, 0, st>>>(*mesh, m_blocks, omega3[0], omega3[1], omega3[2], k_vals);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

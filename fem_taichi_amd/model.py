@@ -184,6 +184,42 @@ class Model:
         n = self.Num_Free
         return tuple(sp.csr_matrix((v.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64) for v in (kt, km))
 
+    # ---- prestressed stiffness for modal analysis (no counterpart in the
+    #      reference; Readme.md:189 lists stress stiffening and spin softening
+    #      as missing) --------------------------------------------------------
+
+    def assemble_KS(self, displace, omega=None, layout="bsr", penalty=True):
+        """K_L + K_sigma(U) - spin softening, values (9*nnzb) on the device:
+        the linear stiffness, the stress-stiffening matrix of the linear
+        stress of the displacement U (fem_hex8_assemble_ks), and, when
+        ``omega`` (rad/s, 3 components) is given, the spin softening
+        -m_jk (|w|^2 I - w w^T) of the rotating frame (fem_spin_softening);
+        then the penalty."""
+        if torch.is_tensor(displace):
+            u = displace.detach().to(device=device(), dtype=torch.float64).reshape(-1).contiguous()
+        else:
+            u = torch.as_tensor(np.asarray(displace, dtype=np.float64).reshape(-1)).to(device())
+        if u.numel() != 3 * self.Num_Node:
+            raise ValueError(f"displacement must have {3 * self.Num_Node} entries")
+        lay = _lib.LAYOUT_BSR if layout == "bsr" else _lib.LAYOUT_CSR
+        k = torch.empty(9 * self.nnzb, dtype=torch.float64, device=device())
+        st = stream_ptr()
+        _lib.call("fem_hex8_assemble_ks", self.Physics.kn_dev.ref, self._young_ptr(), self.young_scalar,
+                  float(self.PO), u.data_ptr(), k.data_ptr(), lay, st)
+        if omega is not None:
+            w = (ctypes.c_double * 3)(*[float(v) for v in omega])
+            _lib.call("fem_spin_softening", self.dev.ref, self.M_blocks().data_ptr(), w, k.data_ptr(), lay, st)
+        if penalty:
+            self.apply_penalty(k, "f64", layout)
+        return k
+
+    def Get_MatrixKS(self, displace, omega=None):
+        """assemble_KS as scipy CSR (3N x 3N), the reference's CSR layout."""
+        k = self.assemble_KS(displace, omega, "csr")
+        row, col = self._csr_index_K()
+        n = self.Num_Free
+        return sp.csr_matrix((k.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64)
+
     def _csr_index_K(self):
         """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173), host copies."""
         d = device()

@@ -109,17 +109,19 @@ class BlockOps:
 class MultigridBlockOps:
     """K (the model's FP64 BSR) and an M-apply, with the inner solves
     K^-1 B done column by column by FP64 flexible CG preconditioned by the
-    FP32 multigrid V-cycle (structured node grids; multigrid.py).  One
-    V-cycle set-up serves every solve of the eigen-solve.  A column stops at
+    FP32 multigrid V-cycle (structured node grids; multigrid.py).  The
+    operator is the model's K or ``k_vals`` (BSR values on the same pattern,
+    e.g. a prestressed K); the V-cycle is built from the model's K either
+    way.  One V-cycle set-up serves every solve of the eigen-solve.  A column stops at
     ``tol`` or at the attainable FP64 accuracy (its true residual no longer
     improving); one that fails otherwise within ``maxiter`` is finished by
     FP64 block-Jacobi PCG."""
 
-    def __init__(self, model, m_apply, min_nodes=1000):
+    def __init__(self, model, m_apply, min_nodes=1000, k_vals=None):
         from . import multigrid, solvers
         from .learned import free_dof_mask
         self.model = model
-        self.K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+        self.K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64") if k_vals is None else k_vals)
         self.pre = multigrid.MultigridPreconditioner(model, multigrid.hierarchy_for(model, min_nodes))
         self.dinv64, _ = self.K64.block_jacobi()
         self.free = free_dof_mask(model)
@@ -204,12 +206,37 @@ class Eigen_solver:
         self.Model = Model
         self.ops = None
 
-    def Solve_cpu(self, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, verbose=False, inner="auto"):
+    def Solve_cpu(self, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, verbose=False, inner="auto",
+                  prestress=None, omega=None):
         """``inner``: "mg" (multigrid flexible CG, structured grids), "pcg"
         (multi-RHS block-Jacobi PCG) or "auto" (mg when the Physics has
-        node_dims)."""
+        node_dims).
+
+        The stiffness operator is K_L; or, after a large-displacement static
+        solve (``Static_solver(Model, 1)``), its final tangent KT, as the
+        reference factors ``Model.KT`` (SOLVE_eigen.py:32); or, when
+        ``prestress`` is a displacement U (3N) or "static" (U from the linear
+        static solve of the model's own load), the prestressed K_L +
+        K_sigma(U) minus the spin softening of ``omega`` (default: the
+        model's Omegas) -- stress stiffening and spin softening, which the
+        reference lists as missing (Readme.md:189)."""
         Mo = self.Model
-        kv = Mo.K_bsr("f64")
+        if prestress is not None:
+            if isinstance(prestress, str):
+                if prestress != "static":
+                    raise ValueError(f"prestress must be a displacement or 'static', got {prestress!r}")
+                from .solve_static import Static_solver
+                structured = getattr(Mo.Physics, "node_dims", None) is not None
+                prestress = Static_solver(Mo, 0).Solve_cpu(method="mg" if structured else "pcg", tol=1e-10)
+            w = np.asarray(Mo.OM if omega is None else omega, dtype=np.float64)
+            kv = Mo.assemble_KS(prestress, omega=w if np.any(w) else None)
+            self.operator = "KS"
+        elif getattr(Mo, "KT_device", None) is not None:
+            kv = Mo.KT_device
+            self.operator = "KT"
+        else:
+            kv = Mo.K_bsr("f64")
+            self.operator = "KL"
         mb = Mo.M_blocks()
         dev_mesh = Mo.dev
         if inner == "auto":
@@ -224,7 +251,7 @@ class Eigen_solver:
                       stream_ptr())
             return Y
 
-        self.ops = MultigridBlockOps(Mo, m_apply) if inner == "mg" else BlockOps(dev_mesh, kv, m_apply)
+        self.ops = MultigridBlockOps(Mo, m_apply, k_vals=kv) if inner == "mg" else BlockOps(dev_mesh, kv, m_apply)
         lam, X, T = lanczos(self.ops, num_eigen, num_block, seed=seed, inner_tol=inner_tol)
         self.eigenvalues = lam
         freq = torch.sqrt(lam) / 2 / np.pi

@@ -17,7 +17,9 @@ max_iters steps.  The update solves KT dU = RF (FP64 block-Jacobi PCG on the
 GPU) and sets U = U - dU.  The reference passes F instead of RF to its
 solve (SOLVE_static.py:70), which from U = 0 steps to U = -K^-1 F and does
 not converge; the residual form is used here and ``self.history`` keeps the
-residual of every step.
+residual of every step.  The final tangent KT stays on the model
+(``Model.KT_device``) for a following modal solve, which then uses it as
+the reference's Eigen_solver uses ``Model.KT``.
 """
 from __future__ import annotations
 
@@ -83,5 +85,8 @@ class Static_solver:
             U = U - dU
             step += 1
         self.U_device = U
+        # the final tangent stays on the model: the modal solve uses it, as the
+        # reference's Eigen_solver factors Model.KT (SOLVE_eigen.py:32)
+        M.KT_device = kt
         self.info = {"steps": step, "residual": self.history[-1]}
         return U.cpu().numpy()

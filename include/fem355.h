@@ -152,6 +152,20 @@ int64_t fem_hex8_assemble_lds_bytes(int32_t max_group_elems, int32_t max_group_r
  * built for this kernel's LDS budget: fem_hex8_kn_lds_bytes). */
 int fem_hex8_assemble_kn(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
                          const double* u, double* kt_vals, double* km_vals, int k_layout, void* stream);
+/* K_L + K_sigma(u): the linear stiffness plus the stress-stiffening
+ * (geometric) matrix of the linear stress of u, K_sigma_jk = sum_g detJ
+ * (a_j^T sigma_g a_k) I with sigma_g = D sym(du/dx) at each Gauss point.  No
+ * counterpart in the reference (its Readme.md:189 lists prestressed modal
+ * analysis as missing); same row groups and limits as fem_hex8_assemble_kn,
+ * whose kernel it shares (F = I, no stress accumulation). */
+int fem_hex8_assemble_ks(const fem_mesh* mesh, const double* young, double young_scalar, double nu,
+                         const double* u, double* k_vals, int k_layout, void* stream);
+/* Spin softening: k_vals -= m_jk (|w|^2 I - w w^T) per node-pair block, m_jk
+ * the consistent-mass scalars of fem_hex8_assemble (m_blocks), w = omega3
+ * (host pointer, 3 doubles, rad/s).  No counterpart in the reference
+ * (Readme.md:189). */
+int fem_spin_softening(const fem_mesh* mesh, const double* m_blocks, const double* omega3,
+                       double* k_vals, int k_layout, void* stream);
 int64_t fem_hex8_kn_lds_bytes(int32_t max_group_elems, int32_t max_group_rows);
 
 /* ---- a8: penalty (replaces ModifyK FEM_model_csr.py:233-242 and

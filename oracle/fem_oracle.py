@@ -29,6 +29,11 @@ lanczos                SOLVE_eigen.py:45-89                        dense scipy e
 element_KN             FEM_model_bsr.py:476-598 (UnitK, BN, S)     U = 0 gives element_K; rigid
                                                                    translation of U invariance;
                                                                    independent F-based closed form
+element_KS             none (Readme.md:189 lists it missing):      U = 0 and rigid U give element_K;
+                       linear prestress stiffening                 affine closed form
+                                                                   v.K_sigma.v = V tr(H s H^T)
+spin_blocks            none (Readme.md:189)                        zz blocks zero for w || z; a rigid
+                                                                   x translation gives -w^2 rho V
 =====================  ==========================================  ==============================
 
 Known reference quirks restated on purpose (see the module docstrings of each
@@ -276,6 +281,49 @@ def element_KN(Coord, CoordU, EX, PO):
     return KT, KM
 
 
+def element_KS(Coord, CoordU, EX, PO):
+    """Prestressed element stiffness of a linear displacement field (no
+    counterpart in the reference, whose Readme.md:189 lists stress
+    stiffening as missing): element_K + K_sigma, (E, 8, 8, 3, 3), with
+        K_sigma[j, k] = sum_g detJ (DN_j^T s_g DN_k) I
+    and s_g the Gauss point's own linear stress D B u (B of _local_B_bsr, the
+    BSR model's strain rows; no accumulation over Gauss points)."""
+    Coord = np.asarray(Coord, dtype=np.float64)
+    CoordU = np.asarray(CoordU, dtype=np.float64)
+    E = Coord.shape[0]
+    D0, D1 = dmat(EX, PO)
+    if D0.ndim == 2:
+        D0 = np.broadcast_to(D0, (E, 3, 3))
+        D1 = np.broadcast_to(D1, (E, 3, 3))
+    K = element_K(Coord, EX, PO).copy()
+    u = np.transpose(CoordU, (0, 2, 1))                              # (E, 8, 3)
+    for g in range(8):
+        DNr = shape_derv(RMAT[g])
+        J = Coord @ DNr
+        det = np.linalg.det(J)
+        DN = DNr[None] @ np.linalg.inv(J)                            # (E, 8, 3)
+        B1, B2 = _local_B_bsr(DN)
+        S0 = np.einsum('eps,es->ep', D0, np.einsum('ejsk,ejk->es', B1, u))
+        S1 = np.einsum('eps,es->ep', D1, np.einsum('ejsk,ejk->es', B2, u))     # yz, xz, xy
+        s = np.zeros((E, 3, 3))
+        s[:, 0, 0], s[:, 1, 1], s[:, 2, 2] = S0[:, 0], S0[:, 1], S0[:, 2]
+        s[:, 0, 1] = s[:, 1, 0] = S1[:, 2]
+        s[:, 0, 2] = s[:, 2, 0] = S1[:, 1]
+        s[:, 1, 2] = s[:, 2, 1] = S1[:, 0]
+        geo = np.einsum('ejp,epq,ekq->ejk', DN, s, DN) * det[:, None, None]
+        K += geo[..., None, None] * np.eye(3)
+    return K
+
+
+def spin_blocks(Mb, OM):
+    """Spin softening per node-pair block (no counterpart in the reference;
+    Readme.md:189): -m_jk (|w|^2 I - w w^T) for the consistent-mass scalars
+    Mb (nnzb,) -> (nnzb, 3, 3)."""
+    w = np.asarray(OM, dtype=np.float64)
+    W = np.dot(w, w) * np.eye(3) - np.outer(w, w)
+    return -np.asarray(Mb, dtype=np.float64)[:, None, None] * W[None]
+
+
 def element_coord_u(U, Elem, Nind):
     """FEM_model_bsr.py:443-446: CoordU[i][k, j] = U[Nind[Elem[j, i]]][k] -> (E, 3, 8)."""
     en = element_nodes(Elem, Nind)
@@ -518,6 +566,18 @@ def model_KN_bsr_values(Node, Elem, pp, EX, PO, Bounds, U):
         V = assemble_K_bsr(Ke, en, pp['SmNode'], pp['ReNode'])
         out.append(modify_K_bsr(V, b, pp['Nind'], pp['SmNode'], pp['ReNode']))
     return out[0], out[1]
+
+
+def model_KS_bsr_values(Node, Elem, pp, EX, PO, Bounds, U, DE=None, OM=None):
+    """K_L + K_sigma(U) (element_KS), minus the spin softening of OM when DE
+    and OM are given, then the penalty: BSR values (nnzb, 3, 3)."""
+    en = element_nodes(Elem, pp['Nind'])
+    Ke = element_KS(element_coords(Node, Elem, pp['Nind']), element_coord_u(U, Elem, pp['Nind']), EX, PO)
+    V = assemble_K_bsr(Ke, en, pp['SmNode'], pp['ReNode'])
+    if OM is not None:
+        Me = element_M(element_coords(Node, Elem, pp['Nind']), DE)
+        V = V + spin_blocks(assemble_M_block(Me, en, pp['SmNode'], pp['ReNode']), OM)
+    return modify_K_bsr(V, stage_bounds(Bounds), pp['Nind'], pp['SmNode'], pp['ReNode'])
 
 
 def model_M_csr(Node, Elem, pp, DE):
