@@ -186,6 +186,11 @@ def time_modal(spec, inner="mg", prestress=False):
     th = time.perf_counter()
     multigrid.hierarchy_for(model)          # symbolic, once per mesh (as the fine pattern)
     hier_s = time.perf_counter() - th
+    kn_s = 0.0
+    if prestress:
+        th = time.perf_counter()
+        phys.kn_dev                         # the stiffening kernel's row groups, once per mesh
+        kn_s = time.perf_counter() - th
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     freq, X = solver.Solve_cpu(num_eigen=modes, num_block=block, seed=0, inner_tol=1e-10, inner=inner,
@@ -202,7 +207,8 @@ def time_modal(spec, inner="mg", prestress=False):
         lc = float(lam[c])
         res.append(float(lc * torch.linalg.vector_norm(Y[:, c] - Xd[:, c] / lc) / torch.linalg.vector_norm(Xd[:, c])))
     out = {"mesh": g.name, "nodes": g.Num_Node, "dof": 3 * g.Num_Node, "modes": modes, "block": block,
-           "inner": inner, "operator": solver.operator, "seconds": secs, "hierarchy_seconds": hier_s, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
+           "inner": inner, "operator": solver.operator, "seconds": secs, "hierarchy_seconds": hier_s,
+           "kn_groups_seconds": kn_s, "inner_iterations": ops.iterations, "freq_hz_first": [float(f) for f in freq[:5]],
            "shift_invert_residual_first8": res}
     print(f"# modal {json.dumps(out)}", flush=True)
     return out
