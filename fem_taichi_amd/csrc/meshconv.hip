@@ -6,11 +6,10 @@
 // (inputs) / 16 (outputs); weights w[tap][co][ci] with ci contiguous, tap =
 // (dz+1)*9 + (dy+1)*3 + (dx+1); out-of-grid neighbours read as zero.
 //
-// Forward (implicit GEMM): a work-group computes 64 pixels x Cout; wave w
-// owns pixels [16w, 16w+16) and Cout/16 accumulator tiles of
-// v_mfma_f32_16x16x32_bf16; the K loop runs over (tap, 32-channel chunk).
-// Operand layout of that instruction: lane l supplies A[l%16][8*(l/16)+i]
-// and B[8*(l/16)+i][l%16], i < 8, and receives D[4*(l/16)+i][l%16], i < 4.
+// Forward (implicit GEMM on v_mfma_f32_16x16x32_bf16, LDS-tiled: see
+// conv3_fwd_tiled_kernel).  Operand layout of that instruction: lane l
+// supplies A[l%16][8*(l/16)+i] and B[8*(l/16)+i][l%16], i < 8, and receives
+// D[4*(l/16)+i][l%16], i < 4.
 // Backward data is the forward kernel on flipped, channel-transposed
 // weights.  The weight gradient is a GEMM over pixels: each work-group owns
 // (tap, pixel chunk), stages 32-pixel tiles of dy and of the shifted input in
@@ -26,7 +25,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kCT = 256;
-constexpr int kTileP = 64;
 
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
@@ -35,50 +33,155 @@ __device__ __forceinline__ bf16x8 zero8() {
   return z;
 }
 
+// ---- LDS-tiled forward: a work-group owns a 16 x 4 x 4 block of output
+// nodes (x fastest); wave w owns the z-plane w of it, i.e. 4 rows of 16 nodes,
+// and all cout channels: 4 x (cout / 16) accumulator tiles, so each A
+// fragment feeds cout / 16 MFMAs and each B fragment 4.  The K loop runs
+// chunk-outer: per 32-channel chunk the 18 x 6 x 6 halo of input nodes is
+// staged in LDS once, the next chunk's halo loaded into registers while this
+// one is multiplied (rows of 32 channels padded to 40, which keeps the
+// 16-byte fragment reads of 16 consecutive nodes conflict-free), then the 27
+// taps run two at a time: each pair's weight slices (cout x 32 channels each)
+// are staged in LDS for all four waves, the next pair loaded into registers
+// while the current one is multiplied, one barrier per pair.
+// The epilogue stages the tile's outputs in LDS and writes whole node rows.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;                  // 648 halo nodes
+constexpr int kLD = 40;                                 // bf16 per staged row (32 + 8 pad)
+constexpr int kTileNodes = kTX * kTY * kTZ;             // 256
+constexpr int kLdsA = kHalo * kLD * 2;                  // 51,840 B
+constexpr int kLdsOut = kTileNodes * 64 * 4;            // f32 staging of 64 channels
+constexpr int kLdsB = 4 * 64 * kLD * 2;                 // two pairs of weight slices
+constexpr int kLdsTiled = (kLdsA + kLdsB > kLdsOut) ? kLdsA + kLdsB : kLdsOut;
+
 template <int NT>
 __global__ void __launch_bounds__(kCT)
-conv3_fwd_kernel(const bf16* __restrict__ x, int cin, const bf16* __restrict__ w,
-                 const float* __restrict__ bias, int cout, int nx, int ny, int nz,
-                 bf16* __restrict__ y_act, bf16* __restrict__ y_pre, float* __restrict__ y_f32, int relu) {
-  const int64_t P = (int64_t)nx * ny * nz;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+conv3_fwd_tiled_kernel(const bf16* __restrict__ x, int cin, const bf16* __restrict__ w,
+                       const float* __restrict__ bias, int cout, int nx, int ny, int nz,
+                       bf16* __restrict__ y_act, bf16* __restrict__ y_pre, float* __restrict__ y_f32, int relu) {
+  __shared__ __align__(16) unsigned char smem[kLdsTiled];
+  bf16* sA = reinterpret_cast<bf16*>(smem);
+  bf16* sB = reinterpret_cast<bf16*>(smem + kLdsA);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m = lane & 15, kq = lane >> 4;
-  const int64_t pbase = (int64_t)blockIdx.x * kTileP + wave * 16;
-  const int64_t p = pbase + m;
-  const bool pin = p < P;
-  const int px = pin ? (int)(p % nx) : 0;
-  const int py = pin ? (int)((p / nx) % ny) : 0;
-  const int pz = pin ? (int)(p / ((int64_t)nx * ny)) : 0;
-  f32x4 acc[NT];
+  const int bx = blockIdx.x * kTX, by = blockIdx.y * kTY, bz = blockIdx.z * kTZ;
+  f32x4 acc[kTY][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int tap = 0; tap < 27; ++tap) {
-    const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
-    const int qx = px + dx, qy = py + dy, qz = pz + dz;
-    const bool valid = pin && qx >= 0 && qx < nx && qy >= 0 && qy < ny && qz >= 0 && qz < nz;
-    const int64_t q = ((int64_t)qz * ny + qy) * nx + qx;
-    for (int c0 = 0; c0 < cin; c0 += 32) {
-      const bf16x8 A = valid ? *reinterpret_cast<const bf16x8*>(x + q * cin + c0 + kq * 8) : zero8();
+  for (int r = 0; r < kTY; ++r)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x8 B =
-            *reinterpret_cast<const bf16x8*>(w + ((int64_t)(tap * cout + t * 16 + m)) * cin + c0 + kq * 8);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+    for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // weight slices (tap, c0) are staged in LDS two taps at a time, double
+  // buffered: slice row co < cout of 4 pieces of 8 channels, thread -> one
+  // piece; the next pair is loaded into registers while the current pair is
+  // multiplied, so one barrier serves two taps.
+  const int brow = tid >> 2, bpart = tid & 3;
+  const bool bload = brow < cout;
+  constexpr int kStage = (kHalo * 4 + kCT - 1) / kCT;    // halo pieces per thread
+  const int64_t wtap = (int64_t)cout * cin;              // stride between taps
+  auto slice = [&](int pb, int k) { return sB + (pb * 2 + k) * (64 * kLD); };
+  // this thread's share of a chunk's halo, loaded into registers (all loads
+  // in flight together); the next chunk's share is loaded during this chunk's taps
+  bf16x8 hv[kStage];
+  auto load_halo = [&](int c) {
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+      const int e = tid + q * kCT;
+      hv[q] = zero8();
+      if (e < kHalo * 4) {
+        const int hn = e >> 2, part = e & 3;
+        const int hx = hn % kHX, hy = (hn / kHX) % kHY, hz = hn / (kHX * kHY);
+        const int gx = bx + hx - 1, gy = by + hy - 1, gz = bz + hz - 1;
+        if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+          hv[q] = *reinterpret_cast<const bf16x8*>(x + (((int64_t)gz * ny + gy) * nx + gx) * cin + c + part * 8);
+      }
+    }
+  };
+  load_halo(0);
+  for (int c0 = 0; c0 < cin; c0 += 32) {
+    const bf16* wc = w + (int64_t)brow * cin + c0 + bpart * 8;
+    const bf16x8 b0 = bload ? *reinterpret_cast<const bf16x8*>(wc) : zero8();
+    const bf16x8 b1 = bload ? *reinterpret_cast<const bf16x8*>(wc + wtap) : zero8();
+    __syncthreads();   // the previous chunk's readers of sA / sB are done
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+      const int e = tid + q * kCT;
+      if (e < kHalo * 4) *reinterpret_cast<bf16x8*>(sA + (e >> 2) * kLD + (e & 3) * 8) = hv[q];
+    }
+    if (bload) {
+      *reinterpret_cast<bf16x8*>(slice(0, 0) + brow * kLD + bpart * 8) = b0;
+      *reinterpret_cast<bf16x8*>(slice(0, 1) + brow * kLD + bpart * 8) = b1;
+    }
+    __syncthreads();
+    if (c0 + 32 < cin) load_halo(c0 + 32);
+    for (int tp = 0; tp < 27; tp += 2) {
+      const int pb = (tp >> 1) & 1;
+      bf16x8 n0 = zero8(), n1 = zero8();
+      if (bload && tp + 2 < 27) n0 = *reinterpret_cast<const bf16x8*>(wc + (tp + 2) * wtap);
+      if (bload && tp + 3 < 27) n1 = *reinterpret_cast<const bf16x8*>(wc + (tp + 3) * wtap);
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const int tap = tp + k;
+        if (tap >= 27) break;
+        const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
+        const bf16* sBb = slice(pb, k);
+        bf16x8 Bf[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) Bf[t] = *reinterpret_cast<const bf16x8*>(sBb + (t * 16 + m) * kLD + kq * 8);
+#pragma unroll
+        for (int r = 0; r < kTY; ++r) {
+          const int hn = ((wave + 1 + dz) * kHY + (r + 1 + dy)) * kHX + (m + 1 + dx);
+          const bf16x8 Af = *reinterpret_cast<const bf16x8*>(sA + hn * kLD + kq * 8);
+#pragma unroll
+          for (int t = 0; t < NT; ++t)
+            acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, Bf[t], acc[r][t], 0, 0, 0);
+        }
+      }
+      if (tp + 2 < 27) {
+        if (bload) {
+          *reinterpret_cast<bf16x8*>(slice(pb ^ 1, 0) + brow * kLD + bpart * 8) = n0;
+          *reinterpret_cast<bf16x8*>(slice(pb ^ 1, 1) + brow * kLD + bpart * 8) = n1;
+        }
+        __syncthreads();   // the next pair visible; this pair's readers done before it is overwritten
       }
     }
   }
+  // ---- epilogue: bias, stage the tile (node-major, cout contiguous) in LDS, write node rows
+  __syncthreads();
+  float* sOf = reinterpret_cast<float*>(smem);
+  bf16* sOb = reinterpret_cast<bf16*>(smem);
+  const bool f32out = y_f32 != nullptr;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int ch = t * 16 + m;
-    const float b = bias ? bias[ch] : 0.f;
+  for (int r = 0; r < kTY; ++r)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int64_t pp = pbase + kq * 4 + i;
-      if (pp >= P) continue;
-      const float v = acc[t][i] + b;
-      if (y_pre) y_pre[pp * cout + ch] = (bf16)v;
-      if (y_f32) y_f32[pp * cout + ch] = v;
-      if (y_act) y_act[pp * cout + ch] = (bf16)((relu && v < 0.f) ? 0.f : v);
+    for (int t = 0; t < NT; ++t) {
+      const int ch = t * 16 + m;
+      const float b = bias ? bias[ch] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int node = (kq * 4 + i) + kTX * (r + kTY * wave);   // tile node index, x fastest
+        const float v = acc[r][t][i] + b;
+        if (f32out) sOf[node * cout + ch] = v;
+        else sOb[node * cout + ch] = (bf16)v;
+      }
+    }
+  __syncthreads();
+  // node (tx, ty, tz) of the tile -> global node; cout values contiguous per node
+  for (int e = tid; e < kTileNodes * cout; e += kCT) {
+    const int node = e / cout, ch = e - node * cout;
+    const int tx = node % kTX, ty = (node / kTX) % kTY, tz = node / (kTX * kTY);
+    const int gx = bx + tx, gy = by + ty, gz = bz + tz;
+    if (gx >= nx || gy >= ny || gz >= nz) continue;
+    const int64_t o = (((int64_t)gz * ny + gy) * nx + gx) * cout + ch;
+    if (f32out) {
+      const float v = sOf[e];
+      y_f32[o] = v;
+      if (y_pre) y_pre[o] = (bf16)v;
+      if (y_act) y_act[o] = (bf16)((relu && v < 0.f) ? 0.f : v);
+    } else {
+      const bf16 v = sOb[e];
+      if (y_pre) y_pre[o] = v;
+      if (y_act) y_act[o] = (relu && (float)v < 0.f) ? (bf16)0.0f : v;
     }
   }
 }
@@ -216,16 +319,17 @@ extern "C" int fem_meshconv_fwd_bf16(const void* x, int cin, const void* w, cons
   int rc = conv_check(cin, cout, nx, ny, nz);
   if (rc) return rc;
   FEM_REQUIRE(x && w && (y_act || y_pre || y_f32), FEM_E_INVALID, "null pointer");
-  const int64_t P = (int64_t)nx * ny * nz;
-  const dim3 grid((unsigned)ceil_div(P, kTileP));
+  const dim3 grid((unsigned)ceil_div(nx, kTX), (unsigned)ceil_div(ny, kTY), (unsigned)ceil_div(nz, kTZ));
+  FEM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, FEM_E_LIMIT, "grid too large for the tiled convolution");
   hipStream_t st = as_stream(stream);
   const bf16* xb = (const bf16*)x;
   const bf16* wb = (const bf16*)w;
+  bf16 *ya = (bf16*)y_act, *yp = (bf16*)y_pre;
   switch (cout / 16) {
-    case 1: conv3_fwd_kernel<1><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
-    case 2: conv3_fwd_kernel<2><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
-    case 3: conv3_fwd_kernel<3><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
-    default: conv3_fwd_kernel<4><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, (bf16*)y_act, (bf16*)y_pre, y_f32, relu); break;
+    case 1: conv3_fwd_tiled_kernel<1><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, ya, yp, y_f32, relu); break;
+    case 2: conv3_fwd_tiled_kernel<2><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, ya, yp, y_f32, relu); break;
+    case 3: conv3_fwd_tiled_kernel<3><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, ya, yp, y_f32, relu); break;
+    default: conv3_fwd_tiled_kernel<4><<<grid, kCT, 0, st>>>(xb, cin, wb, bias, cout, nx, ny, nz, ya, yp, y_f32, relu); break;
   }
   FEM_LAUNCH_CHECK();
   return FEM_OK;
