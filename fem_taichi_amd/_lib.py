@@ -110,9 +110,13 @@ SIGNATURES = {
     "fem_to_bf16": (_I32, [_P, _I64, _I32, _I32, _P, _P]),
     "fem_pcg_replace_residual": (_I32, [_MP, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
     "fem_mg_smooth": (_I32, [_MP, _P, _I32, _P, ctypes.c_float, _P, _P, _P, _I32, _P]),
-    "fem_mg_restrict": (_I32, [_TP, _P, _P, _P]),
-    "fem_mg_prolong_add": (_I32, [_TP, _P, _P, _P]),
+    "fem_mg_restrict": (_I32, [_TP, _P, _P, _I64, _P]),
+    "fem_mg_prolong_add": (_I32, [_TP, _P, _P, _I64, _P]),
     "fem_dense_gemv_f32": (_I32, [_P, _I64, _P, _P, _P]),
+    "fem_mg_smooth_multi": (_I32, [_MP, _P, _P, ctypes.c_float, _P, _P, _P, _I64, _I32, _P]),
+    "fem_dense_gemm_nb_f32": (_I32, [_P, _I64, _P, _P, _I64, _P]),
+    "fem_gemm_tn_work_bytes": (_I64, [_I64, _I64]),
+    "fem_gemm_tn_f64": (_I32, [_P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P]),
 }
 
 _lib = None

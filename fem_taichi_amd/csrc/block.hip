@@ -206,6 +206,94 @@ mpcg_direction_kernel(int64_t N, int nb, const double* __restrict__ Z, double* _
     P[t] = Z[t] + b * P[t];
 }
 
+// C (p x q) = A^T B for tall A (n x p, row stride lda) and B (n x q, row
+// stride ldb): the inner products of block Lanczos (SOLVE_eigen.py:56, 67,
+// 72-73, 76), whose reduction runs over the n = 3N rows.  Work-group g sums
+// its row chunk into a partial p x q block: tiles of R rows are staged in LDS
+// (R from a 48 KB budget); when p q <= 128 the 256 threads split into
+// G = 256 / (p q) row groups (thread t owns output t % pq, rows t / pq
+// (mod G)) folded in group order, else thread t owns outputs t, t + 256, ...
+// The partials are then summed per output in a fixed tree order, so the
+// result is deterministic.
+constexpr int kTT = 256;
+constexpr int kTnLds = 6144;          // doubles of LDS for one A/B row tile
+constexpr int kGridTN = 1024;
+
+template <int NACC>
+__global__ void __launch_bounds__(kTT)
+gemm_tn_partial_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int p, const double* __restrict__ B,
+                       int64_t ldb, int q, int R, int G, double* __restrict__ part) {
+  extern __shared__ double sm[];
+  double* sA = sm;                      // R x p
+  double* sB = sm + R * p;              // R x q
+  const int pq = p * q;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t r0 = (int64_t)blockIdx.x * chunk;
+  const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
+  const int grp = G > 1 ? (int)threadIdx.x / pq : 0;
+  double acc[NACC];
+  int oi[NACC], oj[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) {
+    acc[a] = 0.0;
+    const int o = G > 1 ? (int)threadIdx.x % pq : (int)threadIdx.x + a * kTT;
+    oi[a] = o < pq ? o / q : 0;
+    oj[a] = o < pq ? o % q : 0;
+  }
+  for (int64_t rb = r0; rb < r1; rb += R) {
+    const int nr = (int)(r1 - rb < R ? r1 - rb : R);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * p; e += kTT) {
+      const int r = e / p;
+      sA[e] = A[(rb + r) * lda + (e - r * p)];
+    }
+    for (int e = threadIdx.x; e < nr * q; e += kTT) {
+      const int r = e / q;
+      sB[e] = B[(rb + r) * ldb + (e - r * q)];
+    }
+    __syncthreads();
+    if (grp < G) {
+      for (int r = grp; r < nr; r += G) {
+#pragma unroll
+        for (int a = 0; a < NACC; ++a) acc[a] += sA[r * p + oi[a]] * sB[r * q + oj[a]];
+      }
+    }
+  }
+  if (G > 1) {
+    __syncthreads();
+    sm[threadIdx.x] = grp < G ? acc[0] : 0.0;
+    __syncthreads();
+    if ((int)threadIdx.x < pq) {
+      double s = 0.0;
+      for (int g = 0; g < G; ++g) s += sm[g * pq + threadIdx.x];
+      part[(int64_t)blockIdx.x * pq + threadIdx.x] = s;
+    }
+    return;
+  }
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) {
+    const int o = threadIdx.x + a * kTT;
+    if (o < pq) part[(int64_t)blockIdx.x * pq + o] = acc[a];
+  }
+}
+
+// One work-group per output: thread t sums partials t, t + 256, ... in order,
+// then a fixed LDS tree.
+__global__ void __launch_bounds__(kTT)
+gemm_tn_sum_kernel(const double* __restrict__ part, int pq, int nblk, double* __restrict__ C) {
+  __shared__ double red[kTT];
+  const int o = blockIdx.x;
+  double s = 0.0;
+  for (int g = threadIdx.x; g < nblk; g += kTT) s += part[(int64_t)g * pq + o];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kTT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) C[o] = red[0];
+}
+
 int check_nb(int64_t nb) {
   FEM_REQUIRE(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, FEM_E_INVALID,
               "nrhs must be a power of two in [1, 64] (pad with zero columns), got %lld", (long long)nb);
@@ -252,6 +340,41 @@ extern "C" int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const 
   mspmm_dot_kernel<<<kGridB, kTS, 0, st>>>(*mesh, vals, P, Q, nb, part, state);
   mpcg_update_kernel<<<kGridB, kTS, 0, st>>>(N, nb, X, R, Z, P, Q, dinv, part, part + kGridB * nb, state);
   mpcg_direction_kernel<<<kGridB, kTS, 0, st>>>(N, nb, Z, P, part + kGridB * nb, state);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int64_t fem_gemm_tn_work_bytes(int64_t p, int64_t q) { return 8 * (int64_t)kGridTN * p * q + 256; }
+
+extern "C" int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p,
+                               int64_t q, double* C, void* work, int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(A && B && C && work, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(n >= 0 && p >= 1 && q >= 1 && p <= 512 && q <= 64 && p * q <= 32 * kTT, FEM_E_INVALID,
+              "need 1 <= p <= 512, 1 <= q <= 64, p*q <= %d (got %lld x %lld)", 32 * kTT, (long long)p,
+              (long long)q);
+  FEM_REQUIRE(lda >= p && ldb >= q, FEM_E_INVALID, "row strides lda %lld < p or ldb %lld < q", (long long)lda,
+              (long long)ldb);
+  FEM_REQUIRE(work_bytes >= fem_gemm_tn_work_bytes(p, q), FEM_E_INVALID, "work too small");
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  const int pi = (int)p, qi = (int)q, pqi = (int)(p * q);
+  const int G = pqi <= kTT / 2 ? kTT / pqi : 1;
+  const int nacc = G > 1 ? 1 : (pqi + kTT - 1) / kTT;
+  int R = kTnLds / (pi + qi);
+  if (R > 256) R = 256;
+  const size_t lds = (size_t)R * (pi + qi) * 8;
+  const int nblk = n < 1 ? 1 : (int)(n < kGridTN ? n : kGridTN);
+#define FEM_TN(NA) \
+  gemm_tn_partial_kernel<NA><<<nblk, kTT, lds, st>>>(A, lda, n, pi, B, ldb, qi, R, G, part)
+  if (nacc <= 1) FEM_TN(1);
+  else if (nacc <= 2) FEM_TN(2);
+  else if (nacc <= 4) FEM_TN(4);
+  else if (nacc <= 8) FEM_TN(8);
+  else if (nacc <= 16) FEM_TN(16);
+  else FEM_TN(32);
+#undef FEM_TN
+  FEM_LAUNCH_CHECK();
+  gemm_tn_sum_kernel<<<pqi, kTT, 0, st>>>(part, pqi, nblk, C);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

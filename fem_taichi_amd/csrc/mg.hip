@@ -98,10 +98,12 @@ __device__ __forceinline__ float axis_weight(const int32_t* __restrict__ left, c
 
 // coarse[I] = sum over fine nodes f of P[f, I] fine[f] (thread per coarse node).
 __global__ void __launch_bounds__(kThreads)
-restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __restrict__ coarse) {
-  const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __restrict__ coarse, int nb) {
+  const int64_t tI = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const int64_t ncx = t.nc[0], ncy = t.nc[1];
-  if (I >= ncx * ncy * t.nc[2]) return;
+  if (tI >= ncx * ncy * t.nc[2] * nb) return;
+  const int64_t I = tI / nb;
+  const int col = (int)(tI - I * nb);
   const int cx = (int)(I % ncx), cy = (int)((I / ncx) % ncy), cz = (int)(I / (ncx * ncy));
   const int32_t* lx = t.left;
   const int32_t* ly = lx + t.nf[0];
@@ -120,24 +122,27 @@ restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __re
       const float gyz = gz * axis_weight(ly, wy, fy, cy);
       for (int fx = sx[2 * cx]; fx <= sx[2 * cx + 1]; ++fx) {
         const float g = gyz * axis_weight(lx, wx, fx, cx);
-        const int64_t f = 3 * (fx + fx0 * fy + fxy * fz);
+        const int64_t f = 3 * (fx + fx0 * fy + fxy * fz) * nb + col;
         a0 += g * fine[f];
-        a1 += g * fine[f + 1];
-        a2 += g * fine[f + 2];
+        a1 += g * fine[f + nb];
+        a2 += g * fine[f + 2 * nb];
       }
     }
   }
-  coarse[3 * I] = a0;
-  coarse[3 * I + 1] = a1;
-  coarse[3 * I + 2] = a2;
+  const int64_t o = 3 * I * nb + col;
+  coarse[o] = a0;
+  coarse[o + nb] = a1;
+  coarse[o + 2 * nb] = a2;
 }
 
 // fine[f] += sum over the (at most 8) coarse nodes I of P[f, I] coarse[I].
 __global__ void __launch_bounds__(kThreads)
-prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float* __restrict__ fine) {
-  const int64_t f = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float* __restrict__ fine, int nb) {
+  const int64_t tf = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   const int64_t nfx = t.nf[0], nfy = t.nf[1];
-  if (f >= nfx * nfy * t.nf[2]) return;
+  if (tf >= nfx * nfy * t.nf[2] * nb) return;
+  const int64_t f = tf / nb;
+  const int col = (int)(tf - f * nb);
   const int fx = (int)(f % nfx), fy = (int)((f / nfx) % nfy), fz = (int)(f / (nfx * nfy));
   const int32_t* lx = t.left;
   const int32_t* ly = lx + t.nf[0];
@@ -159,16 +164,75 @@ prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float*
       for (int kx = 0; kx < 2; ++kx) {
         if (gx[kx] == 0.0f) continue;
         const float g = gx[kx] * gy[ky] * gz[kz];
-        const int64_t I = 3 * (cx[kx] + ncx * cy[ky] + ncxy * cz[kz]);
+        const int64_t I = 3 * (cx[kx] + ncx * cy[ky] + ncxy * cz[kz]) * nb + col;
         a0 += g * coarse[I];
-        a1 += g * coarse[I + 1];
-        a2 += g * coarse[I + 2];
+        a1 += g * coarse[I + nb];
+        a2 += g * coarse[I + 2 * nb];
       }
     }
   }
-  fine[3 * f] += a0;
-  fine[3 * f + 1] += a1;
-  fine[3 * f + 2] += a2;
+  const int64_t o = 3 * f * nb + col;
+  fine[o] += a0;
+  fine[o + nb] += a1;
+  fine[o + 2 * nb] += a2;
+}
+
+// ---- several right-hand sides at once (the inner solves of block Lanczos):
+// blocks of vectors are node-interleaved with columns fastest,
+// X[(3 i + p) nb + c], as in the multi-RHS PCG.  Thread per (node, column).
+
+// MODE 0: OUT = B - K X.  MODE 1: OUT = X + w Dinv (B - K X).  MODE 2: OUT = w Dinv B.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __restrict__ dinv, float w,
+                    const float* __restrict__ X, const float* __restrict__ B, float* __restrict__ OUT, int nb) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= m.num_node * nb) return;
+  const int64_t i = t / nb;
+  const int c = (int)(t - i * nb);
+  const int64_t o = 3 * i * nb + c;
+  float r0 = B[o], r1 = B[o + nb], r2 = B[o + 2 * nb];
+  if (MODE != 2) {
+    const int64_t sm = m.sm_node[i];
+    const int co = m.co_node[i];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int b = 0; b < co; ++b) {
+      const float* __restrict__ v = vals + 9 * (sm + b);
+      const int64_t xc = 3 * (int64_t)m.re_node[sm + b] * nb + c;
+      const float x0 = X[xc], x1 = X[xc + nb], x2 = X[xc + 2 * nb];
+      a0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+      a1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+      a2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+    }
+    r0 -= a0;
+    r1 -= a1;
+    r2 -= a2;
+  }
+  if (MODE == 0) {
+    OUT[o] = r0;
+    OUT[o + nb] = r1;
+    OUT[o + 2 * nb] = r2;
+    return;
+  }
+  const float* __restrict__ D = dinv + 9 * i;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const float d = w * (D[3 * p] * r0 + D[3 * p + 1] * r1 + D[3 * p + 2] * r2);
+    OUT[o + p * nb] = MODE == 1 ? X[o + p * nb] + d : d;
+  }
+}
+
+// Y = A X for the coarsest level's dense inverse, X and Y n x nb row-major.
+__global__ void __launch_bounds__(kThreads)
+gemm_nb_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__ X, float* __restrict__ Y, int nb) {
+  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+  if (t >= n * nb) return;
+  const int64_t r = t / nb;
+  const int c = (int)(t - r * nb);
+  const float* __restrict__ a = A + r * n;
+  float s = 0.f;
+  for (int64_t k = 0; k < n; ++k) s += a[k] * X[k * nb + c];
+  Y[t] = s;
 }
 
 // y = A x, A dense n x n row-major; one wave per row.
@@ -228,22 +292,60 @@ extern "C" int fem_mg_smooth(const fem_mesh* mesh, const float* vals, int layout
   return FEM_OK;
 }
 
-extern "C" int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, void* stream) {
+extern "C" int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, int64_t nrhs,
+                               void* stream) {
   int rc = check_transfer(t);
   if (rc) return rc;
   FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
+  FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   const int64_t Nc = (int64_t)t->nc[0] * t->nc[1] * t->nc[2];
-  restrict_kernel<<<ceil_div(Nc, kThreads), kThreads, 0, as_stream(stream)>>>(*t, fine, coarse);
+  restrict_kernel<<<ceil_div(Nc * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(*t, fine, coarse, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
 
-extern "C" int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, void* stream) {
+extern "C" int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, int64_t nrhs,
+                                  void* stream) {
   int rc = check_transfer(t);
   if (rc) return rc;
   FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
+  FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   const int64_t Nf = (int64_t)t->nf[0] * t->nf[1] * t->nf[2];
-  prolong_add_kernel<<<ceil_div(Nf, kThreads), kThreads, 0, as_stream(stream)>>>(*t, coarse, fine);
+  prolong_add_kernel<<<ceil_div(Nf * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(*t, coarse, fine, (int)nrhs);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_mg_smooth_multi(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+                                   const float* X, const float* B, float* OUT, int64_t nrhs, int mode,
+                                   void* stream) {
+  FEM_REQUIRE(mesh && B && OUT, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
+  FEM_REQUIRE(mode >= 0 && mode <= 2, FEM_E_INVALID, "mode must be 0 (residual), 1 (update) or 2 (start)");
+  FEM_REQUIRE(mode == 0 || dinv, FEM_E_INVALID, "null dinv");
+  FEM_REQUIRE(mode == 2 || (vals && X), FEM_E_INVALID, "null vals or X");
+  FEM_REQUIRE(mode != 1 || OUT != X, FEM_E_INVALID, "the update needs OUT != X (X is gathered)");
+  const int64_t T = mesh->num_node * nrhs;
+  if (T == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  const int nb = (int)nrhs;
+  if (mode == 0)
+    smooth_multi_kernel<0><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
+  else if (mode == 1)
+    smooth_multi_kernel<1><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
+  else
+    smooth_multi_kernel<2><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_dense_gemm_nb_f32(const float* A, int64_t n, const float* X, float* Y, int64_t nrhs,
+                                     void* stream) {
+  FEM_REQUIRE(A && X && Y && n >= 0, FEM_E_INVALID, "null pointer or negative size");
+  FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
+  FEM_REQUIRE(X != Y, FEM_E_INVALID, "Y must not alias X");
+  if (n == 0) return FEM_OK;
+  gemm_nb_kernel<<<ceil_div(n * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(A, n, X, Y, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

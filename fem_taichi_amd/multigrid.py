@@ -240,6 +240,7 @@ class MultigridPreconditioner:
         self.x = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
         self.t = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
         self.applications = 0
+        self._multi = {}
         torch.cuda.synchronize()
         self.setup_seconds = time.perf_counter() - t0
 
@@ -298,7 +299,8 @@ class MultigridPreconditioner:
                       self.x[l].data_ptr(), 2, st)
             _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
                       self.t[l].data_ptr(), 0, st)
-            _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, self.t[l].data_ptr(), self.b[l + 1].data_ptr(), st)
+            _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, self.t[l].data_ptr(), self.b[l + 1].data_ptr(), 1,
+                      st)
         n = self.b[L - 1].numel()
         _lib.call("fem_dense_gemv_f32", self.coarse_inv.data_ptr(), n, self.b[L - 1].data_ptr(),
                   self.x[L - 1].data_ptr(), st)
@@ -306,7 +308,7 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, self.x[l + 1].data_ptr(), self.x[l].data_ptr(),
-                      st)
+                      1, st)
             _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
                       self.t[l].data_ptr(), 1, st)
             self.x[l], self.t[l] = self.t[l], self.x[l]
@@ -315,6 +317,37 @@ class MultigridPreconditioner:
     def __call__(self, r: torch.Tensor) -> torch.Tensor:
         self.applications += 1
         return self.vcycle(r.float().contiguous()).double()
+
+    def vcycle_multi(self, B0: torch.Tensor) -> torch.Tensor:
+        """The V-cycle on nb columns at once: B0 (3N, nb) FP32, row-major
+        (node-interleaved, columns fastest); returns X (3N, nb) FP32."""
+        st = stream_ptr()
+        L = self.hier.num_levels
+        nb = int(B0.shape[1])
+        key = ("multi", nb)
+        if key not in self._multi:
+            d = device()
+            n = [3 * int(np.prod(dm)) for dm in self.hier.dims]
+            self._multi[key] = tuple([torch.empty(k, nb, dtype=torch.float32, device=d) for k in n] for _ in range(3))
+        b, x, t = self._multi[key]
+        b[0] = B0.contiguous()
+        for l in range(L - 1):
+            dev = self.meshes[l]
+            k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
+            _lib.call("fem_mg_smooth_multi", dev.ref, None, dv, w, None, b[l].data_ptr(), x[l].data_ptr(), nb, 2, st)
+            _lib.call("fem_mg_smooth_multi", dev.ref, k, dv, w, x[l].data_ptr(), b[l].data_ptr(), t[l].data_ptr(), nb,
+                      0, st)
+            _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, t[l].data_ptr(), b[l + 1].data_ptr(), nb, st)
+        _lib.call("fem_dense_gemm_nb_f32", self.coarse_inv.data_ptr(), b[L - 1].shape[0], b[L - 1].data_ptr(),
+                  x[L - 1].data_ptr(), nb, st)
+        for l in range(L - 2, -1, -1):
+            dev = self.meshes[l]
+            k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
+            _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, x[l + 1].data_ptr(), x[l].data_ptr(), nb, st)
+            _lib.call("fem_mg_smooth_multi", dev.ref, k, dv, w, x[l].data_ptr(), b[l].data_ptr(), t[l].data_ptr(), nb,
+                      1, st)
+            x[l], t[l] = t[l], x[l]
+        return x[0]
 
 
 def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, drop=1e-2, free=None,
@@ -396,6 +429,88 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
     converged = converged and rel < tol
     return x, {"iterations": it, "rel_residual": rel, "converged": converged, "stagnated": stagnated,
                "trace": trace}
+
+
+def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
+                       drop=1e-2):
+    """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
+    FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) and one
+    FP64 SpMM (fem_bsr_spmm) per iteration.  Each column stops on its own
+    (tolerance, or the attainable accuracy as in solve_fcg_mg) and is frozen
+    from then on.  Returns (X, info) with per-column ``converged``,
+    ``stagnated`` and ``rel_residual``."""
+    from .learned import free_dof_mask
+    st = stream_ptr()
+    n, nb = B.shape
+    if free is None:
+        free = free_dof_mask(model)
+    freec = free[:, None]
+    fixed = 1.0 - free
+    if dinv64 is None:
+        dinv64, _ = K64.block_jacobi()
+    d3 = dinv64.view(-1, 3, 3)
+
+    def spmm(X):
+        X = X.contiguous()
+        Y = torch.empty_like(X)
+        _lib.call("fem_bsr_spmm", K64.mesh.ref, K64.vals.data_ptr(), X.data_ptr(), Y.data_ptr(), nb, st)
+        return Y
+
+    B = B.contiguous()
+    bn = torch.linalg.vector_norm(B, dim=0).cpu().numpy()
+    X = torch.zeros_like(B)
+    R = B * freec
+    active = bn > 0
+    converged = ~active
+    stagnated = np.zeros(nb, dtype=bool)
+    anchor = torch.linalg.vector_norm(R, dim=0).cpu().numpy()
+    best = np.full(nb, np.inf)
+    stalls = np.zeros(nb, dtype=int)
+    act_t = torch.as_tensor(active, device=B.device)
+    Pp = Qp = pqp = None
+    it = 0
+    while it < maxiter and active.any():
+        Z = precond.vcycle_multi(R.float()).double() * freec
+        if Pp is None:
+            P = Z
+        else:
+            P = Z - ((Z * Qp).sum(0) / pqp) * Pp
+        Q = spmm(P) * freec
+        pq = (P * Q).sum(0)
+        pq_safe = torch.where(pq != 0, pq, torch.ones_like(pq))
+        alpha = torch.where(act_t, (P * R).sum(0) / pq_safe, torch.zeros_like(pq))
+        X.add_(alpha * P)
+        R.sub_(alpha * Q)
+        Pp, Qp, pqp = P, Q, pq_safe
+        it += 1
+        rf = torch.linalg.vector_norm(R, dim=0).cpu().numpy()
+        if not np.all(np.isfinite(rf[active])):
+            break
+        need = active & ((rf < drop * anchor) | (rf < tol * bn))
+        if need.any():
+            Rt = (B - spmm(X)) * freec
+            tn = torch.linalg.vector_norm(Rt, dim=0).cpu().numpy()
+            sel = torch.as_tensor(need, device=B.device)
+            R = torch.where(sel[None, :], Rt, R)
+            for c in np.nonzero(need)[0]:
+                anchor[c] = tn[c]
+                if tn[c] < tol * bn[c]:
+                    converged[c] = True
+                    active[c] = False
+                elif tn[c] < 0.5 * best[c]:
+                    best[c], stalls[c] = tn[c], 0
+                else:
+                    stalls[c] += 1
+                    if stalls[c] >= patience:
+                        stagnated[c] = True
+                        active[c] = False
+            act_t = torch.as_tensor(active, device=B.device)
+    # the penalised DOFs from their own rows
+    Rp = (B - spmm(X)) * fixed[:, None]
+    X.add_(torch.bmm(d3, Rp.view(-1, 3, nb)).view(n, nb) * fixed[:, None])
+    res = torch.linalg.vector_norm(B - spmm(X), dim=0).cpu().numpy() / np.where(bn > 0, bn, 1.0)
+    converged = converged & (res < tol)
+    return X, {"iterations": it, "rel_residual": res, "converged": converged, "stagnated": stagnated}
 
 
 def solve_mg(model, K64, F, tol=1e-8, maxiter=1000, min_nodes=1000, fallback=True, **kw):

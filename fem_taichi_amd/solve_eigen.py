@@ -14,8 +14,10 @@ the inner solves K^-1 (M u) are iterative GPU solves to ``inner_tol``
 instead of a Pardiso factorisation (FP64 flexible CG with the FP32 multigrid
 V-cycle on structured grids, multi-RHS block-Jacobi PCG otherwise); the start block is drawn from
 ``np.random.default_rng(seed)`` (the reference's is unseeded); the dense
-b x b / m x m work (Cholesky, inverse, eigh of T) and the tall-skinny
-reorthogonalisation products run as FP64 library calls on the device.
+b x b / m x m work (Cholesky, inverse, eigh of T) and the tall-by-small
+products Q C run as FP64 library calls on the device; the inner products
+X^T Y over the 3N rows (the M-Gram blocks and the reorthogonalisation
+coefficients) are fem_gemm_tn_f64, a deterministic row-split reduction.
 Sparse products (K and M) and the PCG are the fem355 HIP kernels.
 """
 from __future__ import annotations
@@ -108,8 +110,9 @@ class BlockOps:
 
 class MultigridBlockOps:
     """K (the model's FP64 BSR) and an M-apply, with the inner solves
-    K^-1 B done column by column by FP64 flexible CG preconditioned by the
-    FP32 multigrid V-cycle (structured node grids; multigrid.py).  The
+    K^-1 B done by FP64 block flexible CG (all columns at once, per-column
+    scalars) preconditioned by the FP32 multigrid V-cycle on all columns
+    (structured node grids; multigrid.py).  The
     operator is the model's K or ``k_vals`` (BSR values on the same pattern,
     e.g. a prestressed K); the V-cycle is built from the model's K either
     way.  One V-cycle set-up serves every solve of the eigen-solve.  A column stops at
@@ -132,19 +135,49 @@ class MultigridBlockOps:
 
     def solve(self, B, tol, maxiter=1000):
         from . import multigrid, solvers
-        X = torch.empty_like(B)
-        total = 0
-        for j in range(B.shape[1]):
-            b = B[:, j].contiguous()
-            x, info = multigrid.solve_fcg_mg(self.model, self.K64, b, self.pre, tol=tol, maxiter=maxiter,
-                                             free=self.free, dinv64=self.dinv64)
-            total += info["iterations"]
-            if not (info["converged"] or info["stagnated"]):
-                x, it2, _ = solvers.pcg(self.K64, b, self.dinv64, tol, 200000, x0=x.clone())
-                total += it2
+        X, info = multigrid.solve_block_fcg_mg(self.model, self.K64, B.contiguous(), self.pre, tol=tol,
+                                               maxiter=maxiter, free=self.free, dinv64=self.dinv64)
+        total = info["iterations"] * B.shape[1]
+        for j in np.nonzero(~(info["converged"] | info["stagnated"]))[0]:
+            x, it2, _ = solvers.pcg(self.K64, B[:, j].contiguous(), self.dinv64, tol, 200000,
+                                    x0=X[:, j].contiguous().clone())
             X[:, j] = x
+            total += it2
         self.iterations.append(total)
         return X
+
+
+_TN_WORK = {}
+
+
+def gram(X, Y):
+    """X^T Y for tall FP64 device matrices X (n, p) and Y (n, q) whose rows are
+    contiguous (fem_gemm_tn_f64; X's columns in pieces of at most 512)."""
+    if X.dtype != torch.float64 or Y.dtype != torch.float64 or X.shape[0] != Y.shape[0]:
+        raise ValueError("gram needs FP64 matrices with the same number of rows")
+    if X.stride(1) != 1:
+        X = X.contiguous()
+    if Y.stride(1) != 1:
+        Y = Y.contiguous()
+    n, p = X.shape
+    q = Y.shape[1]
+    if q > 64:
+        return torch.cat([gram(X, Y[:, j:j + 64]) for j in range(0, q, 64)], dim=1)
+    if p > 512 or p * q > 8192:
+        w = max(1, min(512, 8192 // q))
+        return torch.cat([gram(X[:, i:i + w], Y) for i in range(0, p, w)], dim=0)
+    if n == 0:
+        return torch.zeros(p, q, dtype=torch.float64, device=X.device)
+    lib = _lib.lib()
+    wb = int(lib.fem_gemm_tn_work_bytes(p, q))
+    work = _TN_WORK.get(X.device)
+    if work is None or work.numel() < wb:
+        work = torch.empty(max(wb, 1 << 20), dtype=torch.uint8, device=X.device)
+        _TN_WORK[X.device] = work
+    C = torch.empty(p, q, dtype=torch.float64, device=X.device)
+    _lib.call("fem_gemm_tn_f64", X.data_ptr(), X.stride(0), Y.data_ptr(), Y.stride(0), n, p, q, C.data_ptr(),
+              work.data_ptr(), work.numel(), stream_ptr())
+    return C
 
 
 def _chol_upper(A):
@@ -165,22 +198,22 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     if u0 is None:
         u0 = np.random.default_rng(seed).uniform(size=(n, b))
     u = torch.as_tensor(np.asarray(u0, dtype=np.float64)).to(d)
-    v = _chol_upper(u.T @ ops.m_apply(u))
+    v = _chol_upper(gram(u, ops.m_apply(u)))
     u = u @ torch.linalg.inv(v)
     Q[:, :b] = u
     steps = int(m / b)
     for i in range(steps):
         v = ops.solve(ops.m_apply(u), inner_tol)
-        Tii = u.T @ ops.m_apply(v)
+        Tii = gram(u, ops.m_apply(v))
         T[i * b:(i + 1) * b, i * b:(i + 1) * b] = Tii
         v = v - u @ Tii
         if i >= steps - 1:
             break
         k = i * b + b
         Qk = Q[:, :k]
-        v = v - Qk @ (Qk.T @ ops.m_apply(v))
-        v = v - Qk @ (Qk.T @ ops.m_apply(v))
-        t = _chol_upper(v.T @ ops.m_apply(v))
+        v = v - Qk @ gram(Qk, ops.m_apply(v))
+        v = v - Qk @ gram(Qk, ops.m_apply(v))
+        t = _chol_upper(gram(v, ops.m_apply(v)))
         T[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = t
         T[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = t.T
         u = v @ torch.linalg.inv(t)

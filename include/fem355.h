@@ -284,12 +284,30 @@ typedef struct fem_grid_transfer {
  * not alias x);  mode 2: out = omega Dinv b (vals and x unused). */
 int fem_mg_smooth(const fem_mesh* mesh, const float* vals, int layout, const float* dinv, float omega,
                   const float* x, const float* b, float* out, int mode, void* stream);
-/* coarse = P^T fine (3 FP32 values per node). */
-int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, void* stream);
-/* fine += P coarse. */
-int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, void* stream);
+/* coarse = P^T fine, for nrhs column-interleaved vectors (3 FP32 values per
+ * node and column: v[(3*i + p)*nrhs + c]; nrhs in [1, 64]). */
+int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, float* coarse, int64_t nrhs,
+                    void* stream);
+/* fine += P coarse (same layout). */
+int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coarse, float* fine, int64_t nrhs,
+                       void* stream);
 /* y = A x, A dense n x n FP32 row-major (the coarsest level's inverse). */
 int fem_dense_gemv_f32(const float* A, int64_t n, const float* x, float* y, void* stream);
+/* The smoother modes of fem_mg_smooth (BSR values) on nrhs column-interleaved
+ * vectors at once (the inner solves of block Lanczos). */
+int fem_mg_smooth_multi(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
+                        const float* X, const float* B, float* OUT, int64_t nrhs, int mode, void* stream);
+/* Y = A X, A dense n x n FP32 row-major, X and Y n x nrhs row-major. */
+int fem_dense_gemm_nb_f32(const float* A, int64_t n, const float* X, float* Y, int64_t nrhs, void* stream);
+
+/* C (p x q, row-major) = A^T B for tall A (n x p, row stride lda) and B
+ * (n x q, row stride ldb), FP64: the inner products of block Lanczos
+ * (SOLVE_eigen.py:56, 67, 72-73, 76).  Reduced over row chunks into
+ * partials in work (fem_gemm_tn_work_bytes), then summed in a fixed order
+ * (deterministic).  p <= 512, q <= 64, p*q <= 8192. */
+int64_t fem_gemm_tn_work_bytes(int64_t p, int64_t q);
+int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
+                    double* C, void* work, int64_t work_bytes, void* stream);
 
 /* ---- 3x3x3 mesh convolution on the structured node grid (the learned
  *      preconditioner; no counterpart in the reference) ---------------------

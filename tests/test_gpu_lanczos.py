@@ -130,3 +130,29 @@ def test_eigen_solver_multigrid_inner_matches_pcg_inner(gpu):
           "pcg", s_pcg.ops.iterations[:3], flush=True)
     assert np.allclose(lam_mg[:3], ref[:3], rtol=EIG_TOL, atol=0)
     assert np.allclose(lam_mg[:3], lam_pcg[:3], rtol=EIG_TOL, atol=0)
+
+
+@pytest.mark.parametrize("n,p,q,ld", [(1000, 8, 8, 8), (300000, 40, 8, 56), (7, 3, 5, 3), (50000, 200, 16, 200),
+                                      (20000, 600, 20, 640), (5000, 16, 70, 16), (0, 4, 4, 4), (1, 1, 1, 1)])
+def test_gram_matches_numpy(gpu, n, p, q, ld):
+    """fem_gemm_tn_f64 (through solve_eigen.gram) against numpy's X^T Y, on a
+    row-strided view of X as the Lanczos basis is, including the column
+    splits beyond the kernel's p <= 512 / q <= 64 and empty inputs; two calls
+    give the same bits."""
+    import torch
+    from fem_taichi_amd.solve_eigen import gram
+    print(f"\ngram n {n} p {p} q {q} lda {ld}", flush=True)
+    rng = np.random.default_rng(n + p)
+    Xw = rng.standard_normal((n, ld))
+    Y = rng.standard_normal((n, q)) * 1e3
+    Xd = torch.as_tensor(Xw).cuda()[:, :p]
+    Yd = torch.as_tensor(Y).cuda()
+    C = gram(Xd, Yd).cpu().numpy()
+    ref = Xw[:, :p].T @ Y
+    assert C.shape == (p, q)
+    if n == 0:
+        assert np.all(C == 0)
+        return
+    err = np.abs(C - ref).max() / (np.abs(Xw[:, :p]).max() * np.abs(Y).max() * np.sqrt(n))
+    assert err < 1e-14
+    assert np.array_equal(gram(Xd, Yd).cpu().numpy(), C)
