@@ -37,7 +37,7 @@ __host__ __device__ constexpr int a_stride() { return sizeof(T) == 8 ? 26 : 25; 
 constexpr int kNone = 0x3FFF;
 
 struct AsmLds {
-  int a_off, w_off, wm_off, n_off, sm_off, co_off, total;
+  int a_off, w_off, wm_off, n_off, sm_off, co_off, pre_off, total;
 };
 
 __host__ __device__ inline int align16(int o) { return (o + 15) & ~15; }
@@ -68,6 +68,8 @@ __host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bo
   L.sm_off = o;
   o += rg * (int)sizeof(int32_t);
   L.co_off = o;
+  o += rg * (int)sizeof(int32_t);
+  L.pre_off = o;
   o += rg * (int)sizeof(int32_t);
   L.total = align16(o);
   return L;
@@ -177,13 +179,19 @@ __device__ __forceinline__ double mass_term(const double* __restrict__ s_wm, con
   return ms;
 }
 
+// Contraction off and the one fused multiply-add explicit, so every call site
+// (direct stores of either layout, the staged copy) rounds the same way.
 template <typename T>
 __device__ __forceinline__ void k_block(const T G[9], T lh, T mh, T v[9]) {
+#pragma clang fp contract(off)
   const T tr = mh * (G[0] + G[4] + G[8]);
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) v[3 * p + q] = lh * G[3 * p + q] + mh * G[3 * q + p] + (p == q ? tr : T(0));
+    for (int q = 0; q < 3; ++q) {
+      const T x = fma(lh, G[3 * p + q], mh * G[3 * q + p]);
+      v[3 * p + q] = p == q ? x + tr : x;
+    }
 }
 
 template <int LAYOUT>
@@ -206,12 +214,29 @@ __device__ __forceinline__ void store_block(T* __restrict__ k_vals, const int32_
   for (int idx = 0; idx < 9; ++idx) k_vals[k_addr<LAYOUT>(sm, co, m, idx)] = v[idx];
 }
 
+// One K block into the staging copy of the group's rows (row r's blocks from
+// 9 * s_pre[r] on, in column order, as they lie in the BSR values).
+template <typename T>
+__device__ __forceinline__ void stage_block(T* __restrict__ st, const int32_t* s_pre, int out, const T G[9], T lh,
+                                            T mh) {
+  T v[9];
+  k_block(G, lh, mh, v);
+  T* dst = st + 9 * (s_pre[out >> 8] + (out & 255));
+#pragma unroll
+  for (int idx = 0; idx < 9; ++idx) dst[idx] = v[idx];
+}
+
 // 4 waves per SIMD (<= 128 VGPRs): two work-groups of 512 threads per CU.
+// stage != 0 (BSR, set by the host when every group's blocks fit in the
+// gradient area): a single-pass group stages its finished blocks in LDS over
+// the gradients and copies each row's contiguous BSR run out with 32 lanes,
+// instead of one 72-byte block per lane (one store instruction then touches
+// about 4 cache lines instead of about 36).
 template <typename T, bool WANT_K, bool WANT_M, int LAYOUT>
 __global__ void __launch_bounds__(kAsmThreads, 4)
 hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_scalar,
                      double lhat, double mhat, double rho, T* __restrict__ k_vals,
-                     double* __restrict__ m_blocks, int eg, int rg) {
+                     double* __restrict__ m_blocks, int eg, int rg, int stage) {
   extern __shared__ __align__(16) unsigned char smem[];
   const AsmLds L = asm_lds_layout<T>(eg, rg, WANT_K, WANT_M);
   T* s_a = reinterpret_cast<T*>(smem + L.a_off);
@@ -220,6 +245,7 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   double* s_N = reinterpret_cast<double*>(smem + L.n_off);
   int32_t* s_sm = reinterpret_cast<int32_t*>(smem + L.sm_off);
   int32_t* s_co = reinterpret_cast<int32_t*>(smem + L.co_off);
+  int32_t* s_pre = reinterpret_cast<int32_t*>(smem + L.pre_off);
 
   const int grp = xcd_chunked(blockIdx.x, gridDim.x);
   const int e0 = m.grp_elem_ptr[grp];
@@ -300,6 +326,19 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
   }
   __syncthreads();
 
+  // row offsets of the staging copy: exclusive prefix of the rows' block counts
+  const bool staged = WANT_K && LAYOUT == FEM_LAYOUT_BSR && stage != 0 && nt <= kAsmThreads;   // uniform
+  if (staged && tid < 64) {
+    const int own = tid < nr ? s_co[tid] : 0;
+    int v = own;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(v, o, 64);
+      if (tid >= o) v += u;
+    }
+    if (tid < nr) s_pre[tid] = v - own;
+  }
+
   // ---- phase 2: one lane per task, T-lane butterfly per output block
   const T lh = (T)lhat, mh = (T)mhat;
   const int lane = tid & 63;
@@ -321,13 +360,28 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
       gram<T>(s_a, s_w, c1, G1);
       gram<T>(s_a, s_w, c2, G2);
       if (dual) {   // second output of a dual task: no butterfly, store now
-        if (valid && has_b) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outB, G2, lh, mh);
+        if (!staged && valid && has_b) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outB, G2, lh, mh);
       } else {
 #pragma unroll
         for (int q = 0; q < 9; ++q) G1[q] += G2[q];
       }
       block_allsum(G1, Tn);
-      if (valid && (lane & (Tn - 1)) == 0) store_block<T, LAYOUT>(k_vals, s_sm, s_co, outA, G1, lh, mh);
+      const bool head = valid && (lane & (Tn - 1)) == 0;
+      if (staged) {
+        __syncthreads();   // every wave is done with the gradients: their area becomes the staging copy
+        T* st = reinterpret_cast<T*>(smem + L.a_off);
+        if (head) stage_block<T>(st, s_pre, outA, G1, lh, mh);
+        if (valid && has_b) stage_block<T>(st, s_pre, outB, G2, lh, mh);
+        __syncthreads();
+        for (int r = tid >> 5; r < nr; r += kAsmThreads / 32) {
+          const int n9 = 9 * s_co[r];
+          T* __restrict__ dst = k_vals + 9 * (int64_t)s_sm[r];
+          const T* src = st + 9 * s_pre[r];
+          for (int d = tid & 31; d < n9; d += 32) dst[d] = src[d];
+        }
+      } else if (head) {
+        store_block<T, LAYOUT>(k_vals, s_sm, s_co, outA, G1, lh, mh);
+      }
     }
     if (WANT_M) {
       double m1[1] = {mass_term(s_wm, s_N, c1)};
@@ -354,8 +408,11 @@ int launch_asm(const fem_mesh& m, const double* young, double ys, double nu, dou
                                     hipFuncAttributeMaxDynamicSharedMemorySize, L.total));
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
+  // staging needs the group's rows in one wave (prefix scan) and their blocks inside the gradient area
+  const int stage = K && LAYOUT == FEM_LAYOUT_BSR && rg <= 64 &&
+                    (int64_t)m.max_group_slots * 9 * (int64_t)sizeof(T) <= (int64_t)(L.w_off - L.a_off);
   hipLaunchKernelGGL(kern, dim3((unsigned)m.num_groups), dim3(kAsmThreads), L.total, st, m, young,
-                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg);
+                     ys, lhat, mhat, rho, reinterpret_cast<T*>(k_vals), m_blocks, eg, rg, stage);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

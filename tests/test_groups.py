@@ -26,8 +26,13 @@ def incidences(en, N):
     return ne_ptr, (e * 8 + j)[o]
 
 
-def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512):
-    """Phase 2 of the kernel on element blocks ``blocks_e`` (E, 8, 8, ...)."""
+def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512, co_node=None):
+    """Phase 2 of the kernel on element blocks ``blocks_e`` (E, 8, 8, ...).
+
+    With ``co_node`` (blocks per row) a single-pass group with at most 64 rows
+    takes the staged path: head lanes write their block to the staging copy
+    at (prefix of the group's row block counts)[r] + m, then each row's run is
+    copied out to its BSR slots; every staging entry must be written once."""
     out = np.zeros((nslots,) + blocks_e.shape[3:])
     hits = np.zeros(nslots, dtype=np.int64)
     gep, ge = groups["grp_elem_ptr"], groups["grp_elems"]
@@ -44,6 +49,13 @@ def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512):
 
     for g in range(len(tp) - 1):
         t0, nt = tp[g], tp[g + 1] - tp[g]
+        rows = grows[grp[g]:grp[g + 1]]
+        staged = co_node is not None and nt <= threads and len(rows) <= 64
+        if staged:
+            co = co_node[rows]
+            pre = np.concatenate([[0], np.cumsum(co)[:-1]])
+            stage = np.zeros((int(co.sum()),) + blocks_e.shape[3:])
+            stage_hits = np.zeros(int(co.sum()), dtype=np.int64)
         for base in range(0, nt, threads):
             for w0 in range(0, threads, wave):
                 lanes = range(w0, w0 + wave)
@@ -81,10 +93,19 @@ def emulate(tasks, tp, groups, blocks_e, SmNode, nslots, wave=64, threads=512):
                         continue
                     for o, v in [(outA[li], val[li])] + ([(outB[li], c2v[li])] if dual[li] and outB[li] >= 0 else []):
                         r, m = o >> 8, o & 255
+                        if staged:
+                            stage[pre[r] + m] = v
+                            stage_hits[pre[r] + m] += 1
+                            continue
                         node = grows[grp[g] + r]
                         slot = SmNode[node] + m
                         out[slot] = v
                         hits[slot] += 1
+        if staged:
+            assert np.all(stage_hits == 1)
+            for r, node in enumerate(rows):
+                out[SmNode[node]:SmNode[node] + co[r]] = stage[pre[r]:pre[r] + co[r]]
+                hits[SmNode[node]:SmNode[node] + co[r]] += 1
     return out, hits
 
 
@@ -148,6 +169,9 @@ def test_tasks_assemble_like_oracle(name, kw, brick):
     Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
     err = np.linalg.norm(Kt.reshape(-1) - Kref.reshape(-1)) / np.linalg.norm(Kref)
     assert err < 1e-14, err
+    Ks, hits = emulate(tasks, tp, groups, Ke, pp["SmNode"], nnzb, co_node=pp["CoNode"])
+    assert np.all(hits == 1)
+    assert np.array_equal(Ks, Kt)
 
     Me = O.element_M(Coord, g.Materials["DE"])
     Mt, hits = emulate(tasks, tp, groups, Me, pp["SmNode"], nnzb)
