@@ -294,6 +294,176 @@ gemm_tn_sum_kernel(const double* __restrict__ part, int pq, int nblk, double* __
   if (threadIdx.x == 0) C[o] = red[0];
 }
 
+// ---- flexible CG, FCG(1), vector steps on nb column-interleaved vectors
+// (entry (i, c) at i * nb + c): the Krylov updates around the multigrid
+// preconditioner, fused so that each step reads each vector once.  Work-group
+// b owns rows [b * chunk, (b + 1) * chunk); thread t takes column t % nb and
+// rows t / nb, t / nb + G, ... (G = 256 / nb row subgroups), so a wave reads
+// contiguous memory.  Each dot product leaves one partial per (work-group,
+// column), folded over subgroups in order, and a one-work-group kernel folds
+// the partials in work-group order: deterministic, no atomics.
+constexpr int kFT = 256;
+constexpr int kFcgGrid = 1024;
+// per-column scalars after the partials (doubles)
+enum { kScBeta = 0, kScAlpha = 1, kScPqPrev = 2, kScCount = 4 };
+
+struct FcgMap {
+  int64_t r0, r1;
+  int c, s, G;
+  bool on;
+};
+
+__device__ __forceinline__ FcgMap fcg_map(int64_t n, int nb) {
+  FcgMap f;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  f.r0 = (int64_t)blockIdx.x * chunk;
+  f.r1 = f.r0 + chunk < n ? f.r0 + chunk : n;
+  f.G = kFT / nb;
+  f.c = (int)threadIdx.x % nb;
+  f.s = (int)threadIdx.x / nb;
+  f.on = f.s < f.G;
+  return f;
+}
+
+// Sum of v over the row subgroups of each column, in subgroup order, written
+// to out[c] by thread c (c < nb).
+__device__ __forceinline__ void fcg_block_fold(double v, const FcgMap& f, int nb, double* red, double* out) {
+  __syncthreads();
+  red[threadIdx.x] = f.on ? v : 0.0;
+  __syncthreads();
+  if ((int)threadIdx.x < nb) {
+    double s = 0.0;
+    for (int g = 0; g < f.G; ++g) s += red[g * nb + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// partial[b][c] = sum over the work-group's rows of mask_i * A_ic * B_ic (A FP32 or FP64)
+template <typename TA>
+__global__ void __launch_bounds__(kFT)
+fcg_dot_kernel(int64_t n, int nb, const TA* __restrict__ A, const double* __restrict__ B,
+               const double* __restrict__ mask, double* __restrict__ part) {
+  __shared__ double red[kFT];
+  const FcgMap f = fcg_map(n, nb);
+  double acc = 0.0;
+  if (f.on)
+    for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+      const double a = (double)A[i * nb + f.c] * B[i * nb + f.c];
+      acc += mask ? a * mask[i] : a;
+    }
+  fcg_block_fold(acc, f, nb, red, part + (int64_t)blockIdx.x * nb);
+}
+
+// Two partials at once: sum P*Q and sum P*R.
+__global__ void __launch_bounds__(kFT)
+fcg_dot2_kernel(int64_t n, int nb, const double* __restrict__ P, const double* __restrict__ Q,
+                const double* __restrict__ R, double* __restrict__ part_pq, double* __restrict__ part_pr) {
+  __shared__ double red[kFT];
+  const FcgMap f = fcg_map(n, nb);
+  double pq = 0.0, pr = 0.0;
+  if (f.on)
+    for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+      const int64_t e = i * nb + f.c;
+      const double p = P[e];
+      pq += p * Q[e];
+      pr += p * R[e];
+    }
+  fcg_block_fold(pq, f, nb, red, part_pq + (int64_t)blockIdx.x * nb);
+  fcg_block_fold(pr, f, nb, red, part_pr + (int64_t)blockIdx.x * nb);
+}
+
+// Column totals of nblk partials, in work-group order within each of the
+// G = 256 / nb strided subsets, then over the subsets in order.
+__device__ __forceinline__ double fcg_fold_partials(const double* __restrict__ part, int nblk, int nb, double* red,
+                                                    int* col) {
+  const int G = kFT / nb;
+  const int c = (int)threadIdx.x % nb, s = (int)threadIdx.x / nb;
+  double v = 0.0;
+  if (s < G)
+    for (int g = s; g < nblk; g += G) v += part[(int64_t)g * nb + c];
+  __syncthreads();
+  red[threadIdx.x] = s < G ? v : 0.0;
+  __syncthreads();
+  double tot = 0.0;
+  if ((int)threadIdx.x < nb)
+    for (int g = 0; g < G; ++g) tot += red[g * nb + threadIdx.x];
+  *col = (int)threadIdx.x;
+  return tot;
+}
+
+// beta_c = (sum mask Z Qp)_c / pq_prev_c
+__global__ void __launch_bounds__(kFT)
+fcg_beta_kernel(const double* __restrict__ part, int nblk, int nb, double* __restrict__ sc) {
+  __shared__ double red[kFT];
+  int c;
+  const double zq = fcg_fold_partials(part, nblk, nb, red, &c);
+  if (c < nb) sc[kScBeta * nb + c] = zq / sc[kScPqPrev * nb + c];
+}
+
+// alpha_c = pr / pq on active columns (0 on frozen ones); pq kept for beta
+__global__ void __launch_bounds__(kFT)
+fcg_alpha_kernel(const double* __restrict__ part_pq, const double* __restrict__ part_pr, int nblk, int nb,
+                 const int32_t* __restrict__ active, double* __restrict__ sc) {
+  __shared__ double red[kFT];
+  int c;
+  const double pq = fcg_fold_partials(part_pq, nblk, nb, red, &c);
+  const double pr = fcg_fold_partials(part_pr, nblk, nb, red, &c);
+  if (c < nb) {
+    const double pq_safe = pq != 0.0 ? pq : 1.0;
+    sc[kScAlpha * nb + c] = (active == nullptr || active[c]) ? pr / pq_safe : 0.0;
+    sc[kScPqPrev * nb + c] = pq_safe;
+  }
+}
+
+__global__ void __launch_bounds__(kFT)
+fcg_sum_kernel(const double* __restrict__ part, int nblk, int nb, double* __restrict__ out) {
+  __shared__ double red[kFT];
+  int c;
+  const double v = fcg_fold_partials(part, nblk, nb, red, &c);
+  if (c < nb) out[c] = v;
+}
+
+// P = mask * Z - beta * P   (first: P = mask * Z)
+template <typename TZ>
+__global__ void __launch_bounds__(kFT)
+fcg_dir_kernel(int64_t n, int nb, const TZ* __restrict__ Z, const double* __restrict__ mask,
+               const double* __restrict__ sc, int first, double* __restrict__ P) {
+  const FcgMap f = fcg_map(n, nb);
+  if (!f.on) return;
+  const double beta = first ? 0.0 : sc[kScBeta * nb + f.c];
+  for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+    const int64_t e = i * nb + f.c;
+    double z = (double)Z[e];
+    if (mask) z *= mask[i];
+    P[e] = first ? z : z - beta * P[e];
+  }
+}
+
+// X += alpha P; R -= alpha * mask * Q; R32 = (float)R; partial sum R^2
+__global__ void __launch_bounds__(kFT)
+fcg_update_kernel(int64_t n, int nb, const double* __restrict__ P, const double* __restrict__ Q,
+                  const double* __restrict__ mask, const double* __restrict__ sc, double* __restrict__ X,
+                  double* __restrict__ R, float* __restrict__ R32, double* __restrict__ part) {
+  __shared__ double red[kFT];
+  const FcgMap f = fcg_map(n, nb);
+  double rr = 0.0;
+  if (f.on) {
+    const double alpha = sc[kScAlpha * nb + f.c];
+    for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+      const int64_t e = i * nb + f.c;
+      X[e] += alpha * P[e];
+      const double q = mask ? Q[e] * mask[i] : Q[e];
+      const double r = R[e] - alpha * q;
+      R[e] = r;
+      if (R32) R32[e] = (float)r;
+      rr += r * r;
+    }
+  }
+  fcg_block_fold(rr, f, nb, red, part + (int64_t)blockIdx.x * nb);
+}
+
+int fcg_grid(int64_t n) { return (int)(n < kFcgGrid ? (n < 1 ? 1 : n) : kFcgGrid); }
+
 int check_nb(int64_t nb) {
   FEM_REQUIRE(nb >= 1 && nb <= 64 && (nb & (nb - 1)) == 0, FEM_E_INVALID,
               "nrhs must be a power of two in [1, 64] (pad with zero columns), got %lld", (long long)nb);
@@ -375,6 +545,66 @@ extern "C" int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, in
 #undef FEM_TN
   FEM_LAUNCH_CHECK();
   gemm_tn_sum_kernel<<<pqi, kTT, 0, st>>>(part, pqi, nblk, C);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+// ---- FCG(1) vector steps -------------------------------------------------
+// work: three partial arrays (kFcgGrid x nb doubles each), then the
+// per-column scalars (beta, alpha, the previous p.q), which carry from
+// fem_fcg_update to the next fem_fcg_direction: pass the same work buffer.
+extern "C" int64_t fem_fcg_work_bytes(int64_t nb) { return 8 * (3 * (int64_t)kFcgGrid * nb + kScCount * nb) + 256; }
+
+extern "C" int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dtype, const double* mask,
+                                 const double* Qp, double* P, int first, void* work, int64_t work_bytes,
+                                 void* stream) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64, FEM_E_INVALID, "nb must be in [1, 64] (got %lld)", (long long)nb);
+  FEM_REQUIRE(z_dtype == FEM_DTYPE_F32 || z_dtype == FEM_DTYPE_F64, FEM_E_INVALID, "bad z_dtype %d", z_dtype);
+  FEM_REQUIRE(n >= 0, FEM_E_INVALID, "negative row count");
+  FEM_REQUIRE(work && work_bytes >= fem_fcg_work_bytes(nb), FEM_E_INVALID, "work too small");
+  if (n == 0) return FEM_OK;
+  FEM_REQUIRE(Z && P && (first || Qp), FEM_E_INVALID, "null pointer");
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  double* sc = part + 3 * (int64_t)kFcgGrid * nb;
+  const int grid = fcg_grid(n), nbi = (int)nb;
+  const bool f32 = z_dtype == FEM_DTYPE_F32;
+  const float* Zf = reinterpret_cast<const float*>(Z);
+  const double* Zd = reinterpret_cast<const double*>(Z);
+  if (!first) {
+    if (f32) fcg_dot_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, Qp, mask, part);
+    else fcg_dot_kernel<double><<<grid, kFT, 0, st>>>(n, nbi, Zd, Qp, mask, part);
+    FEM_LAUNCH_CHECK();
+    fcg_beta_kernel<<<1, kFT, 0, st>>>(part, grid, nbi, sc);
+    FEM_LAUNCH_CHECK();
+  }
+  if (f32) fcg_dir_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, mask, sc, first ? 1 : 0, P);
+  else fcg_dir_kernel<double><<<grid, kFT, 0, st>>>(n, nbi, Zd, mask, sc, first ? 1 : 0, P);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_fcg_update(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
+                              const int32_t* active, double* X, double* R, float* R32, double* rr, void* work,
+                              int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64, FEM_E_INVALID, "nb must be in [1, 64] (got %lld)", (long long)nb);
+  FEM_REQUIRE(n >= 0, FEM_E_INVALID, "negative row count");
+  FEM_REQUIRE(work && work_bytes >= fem_fcg_work_bytes(nb), FEM_E_INVALID, "work too small");
+  FEM_REQUIRE(rr, FEM_E_INVALID, "null rr");
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  double* part2 = part + (int64_t)kFcgGrid * nb;
+  double* part3 = part2 + (int64_t)kFcgGrid * nb;
+  double* sc = part + 3 * (int64_t)kFcgGrid * nb;
+  const int grid = fcg_grid(n), nbi = (int)nb;
+  if (n > 0) FEM_REQUIRE(P && Q && X && R, FEM_E_INVALID, "null pointer");
+  fcg_dot2_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, R, part, part2);
+  FEM_LAUNCH_CHECK();
+  fcg_alpha_kernel<<<1, kFT, 0, st>>>(part, part2, grid, nbi, active, sc);
+  FEM_LAUNCH_CHECK();
+  fcg_update_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, mask, sc, X, R, R32, part3);
+  FEM_LAUNCH_CHECK();
+  fcg_sum_kernel<<<1, kFT, 0, st>>>(part3, grid, nbi, rr);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

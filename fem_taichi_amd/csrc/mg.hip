@@ -223,16 +223,40 @@ smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __r
 }
 
 // Y = A X for the coarsest level's dense inverse, X and Y n x nb row-major.
+// One wave per row of A: lane l takes k = l, l + 64, ... (A's row read
+// coalesced, X's rows as contiguous runs of CB columns), CB column sums per
+// lane, then a fixed butterfly over the wave; columns in chunks of CB.
+constexpr int kGemmCB = 8;
 __global__ void __launch_bounds__(kThreads)
 gemm_nb_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__ X, float* __restrict__ Y, int nb) {
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= n * nb) return;
-  const int64_t r = t / nb;
-  const int c = (int)(t - r * nb);
-  const float* __restrict__ a = A + r * n;
-  float s = 0.f;
-  for (int64_t k = 0; k < n; ++k) s += a[k] * X[k * nb + c];
-  Y[t] = s;
+  const int64_t row = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;                       // whole waves leave together
+  const float* __restrict__ a = A + row * n;
+  for (int c0 = 0; c0 < nb; c0 += kGemmCB) {
+    const int cw = nb - c0 < kGemmCB ? nb - c0 : kGemmCB;
+    float s[kGemmCB];
+#pragma unroll
+    for (int q = 0; q < kGemmCB; ++q) s[q] = 0.f;
+    for (int64_t k = lane; k < n; k += 64) {
+      const float ak = a[k];
+      const float* __restrict__ xk = X + k * nb + c0;
+#pragma unroll
+      for (int q = 0; q < kGemmCB; ++q)
+        if (q < cw) s[q] += ak * xk[q];
+    }
+#pragma unroll
+    for (int q = 0; q < kGemmCB; ++q)
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o, 64);
+    if (lane < cw) {
+      float v = s[0];
+#pragma unroll
+      for (int q = 1; q < kGemmCB; ++q)
+        if (lane == q) v = s[q];
+      Y[row * nb + c0 + lane] = v;
+    }
+  }
 }
 
 // y = A x, A dense n x n row-major; one wave per row.
@@ -345,7 +369,7 @@ extern "C" int fem_dense_gemm_nb_f32(const float* A, int64_t n, const float* X, 
   FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   FEM_REQUIRE(X != Y, FEM_E_INVALID, "Y must not alias X");
   if (n == 0) return FEM_OK;
-  gemm_nb_kernel<<<ceil_div(n * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(A, n, X, Y, (int)nrhs);
+  gemm_nb_kernel<<<ceil_div(n, kThreads / 64), kThreads, 0, as_stream(stream)>>>(A, n, X, Y, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

@@ -134,6 +134,39 @@ __global__ void spmm_kernel(fem_mesh m, const double* __restrict__ vals, const d
   Y[(3 * i + 2) * nrhs + c] = a2;
 }
 
+// SpMM with two columns per thread (nrhs even, >= 8): each K block is loaded
+// once for the pair and the X rows are read as 16-byte pairs.  Measured on
+// the 1M-node blade: 16 columns 2.20 -> 1.36 ms, 8 columns 1.25 -> 1.10 ms;
+// at 4 columns the thread-per-column form is faster.  Same sums in the same
+// order as spmm_kernel.
+__global__ void spmm2_kernel(fem_mesh m, const double* __restrict__ vals, const double* __restrict__ X,
+                             double* __restrict__ Y, int nrhs) {
+  const int npair = nrhs >> 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / npair;
+  const int c = 2 * (int)(t - i * npair);
+  if (i >= m.num_node) return;
+  const int64_t sm = m.sm_node[i];
+  const int co = m.co_node[i];
+  double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  for (int b = 0; b < co; ++b) {
+    const double* v = vals + 9 * (sm + b);
+    const int64_t col = m.re_node[sm + b];
+    const double2 x0 = *reinterpret_cast<const double2*>(X + (3 * col) * nrhs + c);
+    const double2 x1 = *reinterpret_cast<const double2*>(X + (3 * col + 1) * nrhs + c);
+    const double2 x2 = *reinterpret_cast<const double2*>(X + (3 * col + 2) * nrhs + c);
+    a0 += v[0] * x0.x + v[1] * x1.x + v[2] * x2.x;
+    a1 += v[3] * x0.x + v[4] * x1.x + v[5] * x2.x;
+    a2 += v[6] * x0.x + v[7] * x1.x + v[8] * x2.x;
+    b0 += v[0] * x0.y + v[1] * x1.y + v[2] * x2.y;
+    b1 += v[3] * x0.y + v[4] * x1.y + v[5] * x2.y;
+    b2 += v[6] * x0.y + v[7] * x1.y + v[8] * x2.y;
+  }
+  *reinterpret_cast<double2*>(Y + (3 * i) * nrhs + c) = make_double2(a0, b0);
+  *reinterpret_cast<double2*>(Y + (3 * i + 1) * nrhs + c) = make_double2(a1, b1);
+  *reinterpret_cast<double2*>(Y + (3 * i + 2) * nrhs + c) = make_double2(a2, b2);
+}
+
 // Mass product with one scalar per node-pair block on each DOF diagonal.
 __global__ void mass_spmm_kernel(fem_mesh m, const double* __restrict__ mb, const double* __restrict__ X,
                                  double* __restrict__ Y, int nrhs) {
@@ -382,6 +415,13 @@ extern "C" int fem_bsr_spmm(const fem_mesh* mesh, const double* vals, const doub
   FEM_REQUIRE(nrhs > 0 && nrhs <= 1024, FEM_E_INVALID, "nrhs must be in [1, 1024]");
   if (mesh->num_node == 0) return FEM_OK;
   FEM_REQUIRE(vals && X && Y, FEM_E_INVALID, "null pointer");
+  if (nrhs >= 8 && nrhs % 2 == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0 &&
+      reinterpret_cast<uintptr_t>(Y) % 16 == 0) {
+    const int64_t T2 = mesh->num_node * (nrhs / 2);
+    spmm2_kernel<<<ceil_div(T2, 256), 256, 0, as_stream(stream)>>>(*mesh, vals, X, Y, (int)nrhs);
+    FEM_LAUNCH_CHECK();
+    return FEM_OK;
+  }
   const int64_t T = mesh->num_node * nrhs;
   spmm_kernel<<<ceil_div(T, 256), 256, 0, as_stream(stream)>>>(*mesh, vals, X, Y, (int)nrhs);
   FEM_LAUNCH_CHECK();

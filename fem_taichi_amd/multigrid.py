@@ -350,6 +350,37 @@ class MultigridPreconditioner:
         return x[0]
 
 
+class FcgSteps:
+    """Device state of the fused FCG(1) vector steps (fem_fcg_direction /
+    fem_fcg_update) for nb column-interleaved vectors of n rows: the
+    dot-product partials, the per-column scalars carried from one step to
+    the next, and the squared residual norms ``rr`` (device, nb)."""
+
+    def __init__(self, n, nb, dev):
+        self.n, self.nb = int(n), int(nb)
+        self.wb = int(_lib.lib().fem_fcg_work_bytes(self.nb))
+        self.work = torch.empty(self.wb, dtype=torch.uint8, device=dev)
+        self.rr = torch.empty(self.nb, dtype=torch.float64, device=dev)
+
+    def direction(self, Z, mask, Qp, P, first):
+        """P = mask Z - beta P (first: P = mask Z); Z FP32 or FP64."""
+        zdt = _lib.DTYPE_F32 if Z.dtype == torch.float32 else _lib.DTYPE_F64
+        _lib.call("fem_fcg_direction", self.n, self.nb, Z.data_ptr(), zdt, _ptr(mask), _ptr(Qp), P.data_ptr(),
+                  1 if first else 0, self.work.data_ptr(), self.wb, stream_ptr())
+
+    def update(self, P, Q, mask, active, X, R, R32):
+        """alpha = p.r / p.q (0 on inactive columns), X += alpha P,
+        R -= alpha mask Q, R32 = (float) R; returns rr = |R|^2 per column."""
+        _lib.call("fem_fcg_update", self.n, self.nb, P.data_ptr(), Q.data_ptr(), _ptr(mask), _ptr(active),
+                  X.data_ptr(), R.data_ptr(), _ptr(R32), self.rr.data_ptr(), self.work.data_ptr(), self.wb,
+                  stream_ptr())
+        return self.rr
+
+
+def _ptr(t):
+    return None if t is None else t.data_ptr()
+
+
 def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, drop=1e-2, free=None,
                  dinv64=None, patience=3, restart=False):
     """FP64 flexible CG (FCG(1)) with a nonlinear or inexact preconditioner,
@@ -381,34 +412,43 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
     x = torch.zeros_like(F)
     if bn == 0.0:
         return x, {"iterations": 0, "rel_residual": 0.0, "converged": True, "trace": []}
+    n = F.numel()
     r = F * free
-    p_prev = q_prev = pq_prev = None
     anchor = float(torch.linalg.vector_norm(r))
+    # the multigrid V-cycle runs on the FP32 copy of r that the update step
+    # writes; any other preconditioner sees the FP64 r
+    fast = isinstance(precond, MultigridPreconditioner)
+    r32 = r.float() if fast else None
+    steps = FcgSteps(n, 1, F.device)
+    p = torch.empty_like(F)
+    q, q_prev = torch.empty_like(F), torch.empty_like(F)
+    first = True
     it = 0
     converged = stagnated = False
     best_true, stalls = math.inf, 0
     trace = []
     while it < maxiter:
-        z = precond(r) * free
-        if p_prev is None:
-            p = z
+        if fast:
+            precond.applications += 1
+            z = precond.vcycle(r32)
         else:
-            p = z - (torch.dot(z, q_prev) / pq_prev) * p_prev
-        q = K64.matvec(p) * free
-        pq = torch.dot(p, q)
-        alpha = torch.dot(p, r) / pq
-        x.add_(alpha * p)
-        r.sub_(alpha * q)
-        p_prev, q_prev, pq_prev = p, q, pq
+            z = precond(r)
+        steps.direction(z, free, None if first else q_prev, p, first)
+        K64.matvec(p, q)
+        rr = steps.update(p, q, free, None, x, r, r32)
+        q, q_prev = q_prev, q
+        first = False
         it += 1
         if it % check_every:
             continue
-        rf = float(torch.linalg.vector_norm(r))
+        rf = math.sqrt(float(rr[0]))
         if not math.isfinite(rf):
             break
         trace.append((it, rf / bn))
         if rf < drop * anchor or rf < tol * bn:
-            r = (F - K64.matvec(x)) * free
+            r.copy_((F - K64.matvec(x)) * free)
+            if fast:
+                r32.copy_(r)
             anchor = float(torch.linalg.vector_norm(r))
             if anchor < tol * bn:
                 converged = True
@@ -421,7 +461,7 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
                     stagnated = True
                     break
             if restart:
-                p_prev = None
+                first = True
     # the penalised DOFs from their own rows
     rp = (F - K64.matvec(x)) * fixed
     x.add_(torch.bmm(d3, rp.view(-1, 3, 1)).view(-1) * fixed)
@@ -434,8 +474,9 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
 def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
                        drop=1e-2):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
-    FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) and one
-    FP64 SpMM (fem_bsr_spmm) per iteration.  Each column stops on its own
+    FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) on the
+    FP32 copy of the residual, one FP64 SpMM (fem_bsr_spmm) and the fused
+    vector steps (FcgSteps) per iteration.  Each column stops on its own
     (tolerance, or the attainable accuracy as in solve_fcg_mg) and is frozen
     from then on.  Returns (X, info) with per-column ``converged``,
     ``stagnated`` and ``rel_residual``."""
@@ -460,30 +501,28 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     bn = torch.linalg.vector_norm(B, dim=0).cpu().numpy()
     X = torch.zeros_like(B)
     R = B * freec
+    R32 = R.float()
     active = bn > 0
     converged = ~active
     stagnated = np.zeros(nb, dtype=bool)
     anchor = torch.linalg.vector_norm(R, dim=0).cpu().numpy()
     best = np.full(nb, np.inf)
     stalls = np.zeros(nb, dtype=int)
-    act_t = torch.as_tensor(active, device=B.device)
-    Pp = Qp = pqp = None
+    act_d = torch.as_tensor(active.astype(np.int32), device=B.device)
+    steps = FcgSteps(n, nb, B.device)
+    P = torch.empty_like(B)
+    Q, Qp = torch.empty_like(B), torch.empty_like(B)
+    first = True
     it = 0
     while it < maxiter and active.any():
-        Z = precond.vcycle_multi(R.float()).double() * freec
-        if Pp is None:
-            P = Z
-        else:
-            P = Z - ((Z * Qp).sum(0) / pqp) * Pp
-        Q = spmm(P) * freec
-        pq = (P * Q).sum(0)
-        pq_safe = torch.where(pq != 0, pq, torch.ones_like(pq))
-        alpha = torch.where(act_t, (P * R).sum(0) / pq_safe, torch.zeros_like(pq))
-        X.add_(alpha * P)
-        R.sub_(alpha * Q)
-        Pp, Qp, pqp = P, Q, pq_safe
+        Z = precond.vcycle_multi(R32)
+        steps.direction(Z, free, None if first else Qp, P, first)
+        _lib.call("fem_bsr_spmm", K64.mesh.ref, K64.vals.data_ptr(), P.data_ptr(), Q.data_ptr(), nb, st)
+        rr = steps.update(P, Q, free, act_d, X, R, R32)
+        Q, Qp = Qp, Q
+        first = False
         it += 1
-        rf = torch.linalg.vector_norm(R, dim=0).cpu().numpy()
+        rf = np.sqrt(rr.cpu().numpy())
         if not np.all(np.isfinite(rf[active])):
             break
         need = active & ((rf < drop * anchor) | (rf < tol * bn))
@@ -491,7 +530,8 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
             Rt = (B - spmm(X)) * freec
             tn = torch.linalg.vector_norm(Rt, dim=0).cpu().numpy()
             sel = torch.as_tensor(need, device=B.device)
-            R = torch.where(sel[None, :], Rt, R)
+            R.copy_(torch.where(sel[None, :], Rt, R))
+            R32.copy_(R)
             for c in np.nonzero(need)[0]:
                 anchor[c] = tn[c]
                 if tn[c] < tol * bn[c]:
@@ -504,7 +544,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
                     if stalls[c] >= patience:
                         stagnated[c] = True
                         active[c] = False
-            act_t = torch.as_tensor(active, device=B.device)
+            act_d.copy_(torch.as_tensor(active.astype(np.int32)))
     # the penalised DOFs from their own rows
     Rp = (B - spmm(X)) * fixed[:, None]
     X.add_(torch.bmm(d3, Rp.view(-1, 3, nb)).view(n, nb) * fixed[:, None])

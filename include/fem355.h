@@ -309,6 +309,27 @@ int64_t fem_gemm_tn_work_bytes(int64_t p, int64_t q);
 int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
                     double* C, void* work, int64_t work_bytes, void* stream);
 
+/* Flexible CG, FCG(1), vector steps on nb column-interleaved vectors (n rows,
+ * entry (i, c) at i*nb + c), around an outside preconditioner (the multigrid
+ * V-cycle).  The reference's static and shift-invert solves are direct
+ * (SOLVE_static.py:12-38, SOLVE_eigen.py:51); this is the iterative solver
+ * that replaces them.  mask (n doubles, 1 on free DOFs, 0 on penalised ones)
+ * may be NULL (all free).  work (fem_fcg_work_bytes) holds the deterministic
+ * dot-product partials and the per-column scalars that carry from one
+ * fem_fcg_update to the next fem_fcg_direction: pass the same buffer.
+ *   direction: P = mask*Z - beta*P, beta = (mask Z . Qp) / (previous p.q);
+ *              first != 0: P = mask*Z.  Z is FEM_DTYPE_F32 or _F64.
+ *   update:    alpha = p.r / p.q on columns with active[c] != 0 (NULL: all),
+ *              0 elsewhere; X += alpha P; R -= alpha mask*Q; R32 = (float)R
+ *              (R32 may be NULL); rr[c] = |R_c|^2 (device array of nb).
+ * nb <= 64. */
+int64_t fem_fcg_work_bytes(int64_t nb);
+int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dtype, const double* mask, const double* Qp,
+                      double* P, int first, void* work, int64_t work_bytes, void* stream);
+int fem_fcg_update(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
+                   const int32_t* active, double* X, double* R, float* R32, double* rr, void* work,
+                   int64_t work_bytes, void* stream);
+
 /* ---- 3x3x3 mesh convolution on the structured node grid (the learned
  *      preconditioner; no counterpart in the reference) ---------------------
  * BF16 operands on MFMA, FP32 accumulation.  Activations channels-last

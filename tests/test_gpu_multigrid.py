@@ -192,3 +192,83 @@ def test_mg_static_solver_method(gpu):
     Kref = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
     Fref = O.model_F(g.Node, g.Elem, pp, g.DE, g.Omegas, g.Forces)
     assert O.static_residual(Kref, U, Fref) < 1e-9
+
+
+@pytest.mark.parametrize("n", [1, 5, 700, 2049])
+@pytest.mark.parametrize("nb", [1, 3, 8, 13, 16, 64])
+def test_coarsest_dense_products(gpu, n, nb):
+    """fem_dense_gemm_nb_f32 (the coarsest level's inverse on nb columns, in
+    chunks of 8 columns) and fem_dense_gemv_f32 against numpy in FP64."""
+    import torch
+    from fem_taichi_amd import _lib
+    from fem_taichi_amd.runtime import stream_ptr
+    rng = np.random.default_rng(n * 100 + nb)
+    A = rng.standard_normal((n, n)).astype(np.float32)
+    X = rng.standard_normal((n, nb)).astype(np.float32)
+    Ad, Xd = torch.as_tensor(A).cuda(), torch.as_tensor(X).cuda()
+    Y = torch.full((n, nb), np.nan, dtype=torch.float32, device="cuda")
+    _lib.call("fem_dense_gemm_nb_f32", Ad.data_ptr(), n, Xd.data_ptr(), Y.data_ptr(), nb, stream_ptr())
+    ref = A.astype(np.float64) @ X.astype(np.float64)
+    assert rel(Y.cpu().numpy(), ref) < 1e-5
+    y = torch.full((n,), np.nan, dtype=torch.float32, device="cuda")
+    x = Xd[:, 0].contiguous()
+    _lib.call("fem_dense_gemv_f32", Ad.data_ptr(), n, x.data_ptr(), y.data_ptr(), stream_ptr())
+    assert rel(y.cpu().numpy(), ref[:, 0]) < 1e-5
+
+
+@pytest.mark.parametrize("n,nb", [(7, 1), (1000, 1), (77777, 1), (30000, 8), (5001, 10), (20000, 16), (300, 64)])
+def test_fcg_steps_match_numpy(gpu, n, nb):
+    """fem_fcg_direction / fem_fcg_update (through FcgSteps) against the FCG(1)
+    formulas in numpy FP64: two iterations (the first without beta), a masked
+    row set, one frozen column; FP32 and FP64 Z; R32 = float(R); results
+    reproducible bit for bit."""
+    import torch
+    from fem_taichi_amd import multigrid as MG
+    print(f"\nfcg steps n {n} nb {nb}", flush=True)
+    rng = np.random.default_rng(n + nb)
+    mask = (rng.random(n) > 0.1).astype(np.float64)
+    active = np.ones(nb, dtype=np.int32)
+    active[nb // 2] = 0 if nb > 1 else 1
+    X0, R0 = rng.standard_normal((n, nb)), rng.standard_normal((n, nb)) * mask[:, None]
+    Z1, Z2 = rng.standard_normal((n, nb)).astype(np.float32), rng.standard_normal((n, nb))
+    Q1, Q2 = rng.standard_normal((n, nb)), rng.standard_normal((n, nb))
+
+    def ref():
+        X, R = X0.copy(), R0.copy()
+        P = mask[:, None] * Z1.astype(np.float64)
+        pq = (P * Q1).sum(0)
+        alpha = np.where(active > 0, (P * R).sum(0) / pq, 0.0)
+        X += alpha * P
+        R -= alpha * mask[:, None] * Q1
+        zm = mask[:, None] * Z2
+        beta = (zm * Q1).sum(0) / pq
+        P = zm - beta * P
+        pq2 = (P * Q2).sum(0)
+        alpha2 = np.where(active > 0, (P * R).sum(0) / pq2, 0.0)
+        X += alpha2 * P
+        R -= alpha2 * mask[:, None] * Q2
+        return P, X, R, (R * R).sum(0)
+
+    def run():
+        c = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+        st = MG.FcgSteps(n, nb, "cuda")
+        X, R, P = c(X0), c(R0), torch.empty(n, nb, dtype=torch.float64, device="cuda")
+        R32 = torch.empty(n, nb, dtype=torch.float32, device="cuda")
+        m, act, q1, q2 = c(mask), c(active), c(Q1), c(Q2)
+        st.direction(c(Z1), m, None, P, True)
+        st.update(P, q1, m, act, X, R, R32)
+        st.direction(c(Z2), m, q1, P, False)
+        rr = st.update(P, q2, m, act, X, R, R32).cpu().numpy()
+        return P.cpu().numpy(), X.cpu().numpy(), R.cpu().numpy(), rr, R32.cpu().numpy()
+
+    Pr, Xr, Rr, rrr = ref()
+    P, X, R, rr, R32 = run()
+    for a, b in ((P, Pr), (X, Xr), (R, Rr)):
+        assert rel(a, b) < 1e-12
+    assert np.allclose(rr, rrr, rtol=1e-12, atol=0)
+    assert np.array_equal(R32, R.astype(np.float32))
+    if nb > 1:
+        assert np.array_equal(X[:, nb // 2], X0[:, nb // 2])
+    again = run()
+    for a, b in zip(again, (P, X, R, rr, R32)):
+        assert np.array_equal(a, b)

@@ -183,10 +183,10 @@ def test_static_mixed_fallback(setup):
     assert O.static_residual(Kref, U.cpu().numpy(), Fref) < 1e-9
 
 
-@pytest.mark.parametrize("nrhs", [1, 3, 8, 16, 64])
+@pytest.mark.parametrize("nrhs", [1, 3, 8, 10, 16, 64])
 def test_spmm_multi(setup, nrhs):
-    """fem_bsr_spmm / fem_mass_spmm (wave-per-row kernel for power-of-two
-    column counts, thread per (row, column) otherwise) against scipy."""
+    """fem_bsr_spmm (two columns per thread for even counts from 8 on, thread
+    per (row, column) otherwise) and fem_mass_spmm against scipy."""
     import torch
     from fem_taichi_amd import _lib
     from fem_taichi_amd.runtime import stream_ptr
