@@ -90,8 +90,10 @@ def test_smoother_modes(gpu, layout):
     assert rel(out.cpu().numpy(), w * (Dinv @ b.astype(np.float64))) < 1e-5
 
 
-def test_smoother_multi_matches_single(gpu):
-    """fem_mg_smooth_multi on nb columns equals fem_mg_smooth per column."""
+@pytest.mark.parametrize("nb", [3, 8, 16, 18])
+def test_smoother_multi_matches_single(gpu, nb):
+    """fem_mg_smooth_multi on nb columns (1 or, from 16 even columns on, 2
+    columns per thread) equals fem_mg_smooth per column."""
     import torch
     from fem_taichi_amd import Model, Physics, _lib, solvers
     from fem_taichi_amd.runtime import stream_ptr
@@ -100,7 +102,7 @@ def test_smoother_multi_matches_single(gpu):
     model = Model(g.name, phys, g.Materials, Omegas=g.Omegas)
     k32 = model.K_bsr("f32")
     dinv, _ = solvers.BsrOperator(model.dev, k32).block_jacobi()
-    n, nb = 3 * phys.Num_Node, 6
+    n = 3 * phys.Num_Node
     rng = np.random.default_rng(3)
     X = torch.as_tensor(rng.standard_normal((n, nb)).astype(np.float32)).cuda()
     B = torch.as_tensor(rng.standard_normal((n, nb)).astype(np.float32)).cuda() * 1e3
