@@ -249,8 +249,8 @@ class MultigridPreconditioner:
         """Centre of the Chebyshev interval from lambda_max(Dinv K) by power iteration."""
         st = stream_ptr()
         n = 3 * dev.num_node
-        g = torch.Generator(device="cpu").manual_seed(seed)
-        x = torch.randn(n, generator=g, dtype=torch.float32).to(device())
+        g = torch.Generator(device=device()).manual_seed(seed)
+        x = torch.randn(n, generator=g, dtype=torch.float32, device=device())
         x /= torch.linalg.vector_norm(x)
         zero = torch.zeros_like(x)
         r = torch.empty_like(x)
