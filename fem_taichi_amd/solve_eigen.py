@@ -59,6 +59,20 @@ class PatternMesh:
         return ctypes.byref(self.c)
 
 
+MAX_BLOCK = 64      # columns per multi-RHS kernel call
+
+
+def _in_pieces(ops, B, tol, maxiter):
+    """ops.solve on column pieces of at most MAX_BLOCK; one iteration record
+    (the pieces' sum) per call."""
+    parts, total = [], 0
+    for j in range(0, B.shape[1], MAX_BLOCK):
+        parts.append(ops.solve(B[:, j:j + MAX_BLOCK].contiguous(), tol, maxiter))
+        total += ops.iterations.pop()
+    ops.iterations.append(total)
+    return torch.cat(parts, dim=1)
+
+
 class BlockOps:
     """K (BSR 3x3 f64 on a mesh pattern) and an M-apply, with block PCG."""
 
@@ -76,7 +90,10 @@ class BlockOps:
         self.device = device()
 
     def solve(self, B, tol, maxiter=100000, check_every=16):
-        """X = K^-1 B for an (n, b) block by multi-RHS block-Jacobi PCG."""
+        """X = K^-1 B for an (n, b) block by multi-RHS block-Jacobi PCG
+        (blocks wider than the kernels' 64 columns in pieces)."""
+        if B.shape[1] > MAX_BLOCK:
+            return _in_pieces(self, B, tol, maxiter)
         n, b = B.shape
         nb = _pow2(b)
         d = device()
@@ -135,6 +152,8 @@ class MultigridBlockOps:
 
     def solve(self, B, tol, maxiter=1000):
         from . import multigrid, solvers
+        if B.shape[1] > MAX_BLOCK:
+            return _in_pieces(self, B, tol, maxiter)
         X, info = multigrid.solve_block_fcg_mg(self.model, self.K64, B.contiguous(), self.pre, tol=tol,
                                                maxiter=maxiter, free=self.free, dinv64=self.dinv64)
         total = info["iterations"] * B.shape[1]

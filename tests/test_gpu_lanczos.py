@@ -156,3 +156,28 @@ def test_gram_matches_numpy(gpu, n, p, q, ld):
     err = np.abs(C - ref).max() / (np.abs(Xw[:, :p]).max() * np.abs(Y).max() * np.sqrt(n))
     assert err < 1e-14
     assert np.array_equal(gram(Xd, Yd).cpu().numpy(), C)
+
+
+@pytest.mark.parametrize("inner", ["mg", "pcg"])
+def test_block_solve_wider_than_kernels(gpu, inner):
+    """A block of 70 right-hand sides (wider than the 64-column kernels) is
+    solved in pieces: every column's FP64 residual on the oracle's K below
+    1e-8, one iteration record for the call."""
+    import torch
+    from fem_taichi_amd import Model, Physics
+    from fem_taichi_amd.solve_eigen import BlockOps, MultigridBlockOps
+    g = mesh.structured_block(6, 5, 8, 1.0, 0.05, 1)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=(0, 0, 0))
+    pp = O.pre_process(g.Node, g.Elem)
+    K = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
+    n = K.shape[0]
+    B = np.random.default_rng(7).standard_normal((n, 70))
+    if inner == "mg":
+        ops = MultigridBlockOps(model, None, min_nodes=60)
+    else:
+        ops = BlockOps(model.dev, model.K_bsr("f64"), None)
+    X = ops.solve(torch.as_tensor(B).cuda(), 1e-10).cpu().numpy()
+    assert X.shape == (n, 70) and len(ops.iterations) == 1
+    res = np.linalg.norm(B - K @ X, axis=0) / np.linalg.norm(B, axis=0)
+    assert res.max() < 1e-8, res.max()
