@@ -213,3 +213,122 @@ def build_tasks(N, elem, sm_node, re_node, ne_ptr, ne_list, groups):
     tasks = tasks.astype(np.uint32).view(np.int32)
     grp_task_ptr = np.concatenate([[0], np.cumsum(np.bincount(gg, minlength=G))]).astype(np.int32)
     return grp_task_ptr, np.ascontiguousarray(tasks)
+
+
+def build_tasks_torch(N, elem, sm_node, re_node, ne_ptr, ne_list, groups, device="cpu"):
+    """build_tasks with torch on ``device`` (the GPU when there is one): the
+    same task lists, element for element, from sorts and searches that run in
+    milliseconds on the device instead of a minute of host lexsort on a
+    million-node mesh.  Inputs and outputs are numpy arrays as for
+    build_tasks."""
+    import torch
+    d = torch.device(device)
+    i64 = torch.int64
+
+    def t(a):
+        return torch.as_tensor(np.ascontiguousarray(a)).to(d, i64)
+
+    def first_flags(x):
+        f = torch.ones_like(x, dtype=torch.bool)
+        if x.numel() > 1:
+            f[1:] = x[1:] != x[:-1]
+        return f
+
+    ne_ptr_t, elem_t, sm_t = t(ne_ptr), t(elem), t(sm_node)
+    G = int(groups["num_groups"])
+    grp_row_ptr, grp_rows = t(groups["grp_row_ptr"]), t(groups["grp_rows"])
+    nrow = grp_row_ptr[1:] - grp_row_ptr[:-1]
+    if int(nrow.max()) > 255 if nrow.numel() else False:
+        raise ValueError("row groups of more than 255 rows")
+    gid = torch.repeat_interleave(torch.arange(G, device=d), nrow)
+    g_of_node = torch.empty(N, dtype=i64, device=d)
+    r_of_node = torch.empty(N, dtype=i64, device=d)
+    g_of_node[grp_rows] = gid
+    r_of_node[grp_rows] = torch.arange(N, device=d) - grp_row_ptr[gid]
+    inc_node = torch.repeat_interleave(torch.arange(N, device=d), ne_ptr_t[1:] - ne_ptr_t[:-1])
+    ne_list_t = t(ne_list)
+    inc_e, inc_j = ne_list_t >> 3, ne_list_t & 7
+    inc_le = t(groups["ne_local"])
+    if inc_le.numel() and int(inc_le.max()) > 254:
+        raise ValueError("more than 255 elements in a row group")
+    I = inc_node.numel()
+    node = torch.repeat_interleave(inc_node, 8)
+    k = torch.arange(8, device=d).repeat(I)
+    e = torch.repeat_interleave(inc_e, 8)
+    col = elem_t[e, k]
+    rows = torch.repeat_interleave(torch.arange(N, device=d), sm_t[1:] - sm_t[:-1])
+    keys = rows * N + t(re_node)
+    slot = torch.searchsorted(keys, node * N + col)
+    m = slot - sm_t[node]
+    if m.numel() and int(m.max()) > 255:
+        raise ValueError("a row with more than 256 neighbour blocks")
+    g = g_of_node[node]
+    r = r_of_node[node]
+    cc = (torch.repeat_interleave(inc_le, 8) << 6) | (torch.repeat_interleave(inc_j, 8) << 3) | k
+    del node, col, slot, rows, keys
+    # lexsort by (g, r, m), then e: a stable sort on e, then a stable sort on the (g, r, m) key
+    o1 = torch.sort(e, stable=True).indices
+    o2 = torch.sort(((g << 16) | (r << 8) | m)[o1], stable=True).indices
+    order = o1[o2]
+    g, r, m, cc = g[order], r[order], m[order], cc[order]
+    del e, order, o1, o2
+    out_key = (r << 8) | m
+    skey = g * 65536 + out_key
+    start = torch.nonzero(first_flags(skey)).squeeze(1)
+    n = torch.diff(torch.cat([start, torch.tensor([skey.numel()], device=d)]))
+    if n.numel() and int(n.max()) > 64:
+        raise ValueError("a node pair shared by more than 64 elements")
+    half = torch.clamp((n + 1) // 2, min=1)
+    T = torch.ones_like(half)
+    while bool((T < half).any()):
+        T = torch.where(T < half, T * 2, T)
+    sg, sout = g[start], out_key[start]
+    NONE = 0x3FFF
+    recs = []
+    multi = n >= 2
+    ms = torch.nonzero(multi).squeeze(1)
+    if ms.numel():
+        Tm = T[ms]
+        tb = torch.repeat_interleave(ms, Tm)
+        q = torch.arange(int(Tm.sum()), device=d) - torch.repeat_interleave(torch.cumsum(Tm, 0) - Tm, Tm)
+        i1 = start[tb] + 2 * q
+        i2 = i1 + 1
+        has1 = 2 * q < n[tb]
+        has2 = 2 * q + 1 < n[tb]
+        last = cc.numel() - 1
+        c1 = torch.where(has1, cc[torch.clamp(i1, max=last)], torch.full_like(i1, NONE))
+        c2 = torch.where(has2, cc[torch.clamp(i2, max=last)], torch.full_like(i2, NONE))
+        lt = torch.round(torch.log2(T[tb].double())).to(i64)
+        w2 = c1 | (c2 << 14) | (lt << 28)
+        sortk = (20 - lt) * (1 << 24) + tb
+        recs.append((sg[tb], sortk * 64 + q, sout[tb], torch.full_like(tb, -1), w2))
+    ss = torch.nonzero(~multi).squeeze(1)
+    if ss.numel():
+        gs = sg[ss]
+        o = torch.sort(gs, stable=True).indices
+        ss, gs = ss[o], gs[o]
+        ar = torch.arange(ss.numel(), device=d)
+        pos = ar - torch.cummax(torch.where(first_flags(gs), ar, torch.zeros_like(ar)), 0).values
+        a_idx = torch.nonzero(pos % 2 == 0).squeeze(1)
+        b_idx = a_idx + 1
+        lastb = ss.numel() - 1
+        b_ok = (b_idx <= lastb) & (gs[torch.clamp(b_idx, max=lastb)] == gs[a_idx])
+        cA = cc[start[ss[a_idx]]]
+        cB = torch.where(b_ok, cc[start[ss[torch.clamp(b_idx, max=lastb)]]], torch.full_like(a_idx, NONE))
+        oB = torch.where(b_ok, sout[ss[torch.clamp(b_idx, max=lastb)]], torch.full_like(a_idx, -1))
+        w2 = cA | (cB << 14) | (1 << 31)
+        sortk = torch.full_like(a_idx, 21 << 30) + a_idx
+        recs.append((gs[a_idx], sortk, sout[ss[a_idx]], oB, w2))
+    gg = torch.cat([x[0] for x in recs])
+    sk = torch.cat([x[1] for x in recs])
+    oa = torch.cat([x[2] for x in recs])
+    ob = torch.cat([x[3] for x in recs])
+    w2 = torch.cat([x[4] for x in recs])
+    o1 = torch.sort(sk, stable=True).indices
+    o = o1[torch.sort(gg[o1], stable=True).indices]
+    w0 = oa[o] | (torch.where(ob[o] < 0, torch.full_like(ob, 0xFFFF)[o], ob[o]) << 16)
+    tasks = torch.stack([w0, w2[o]], dim=1) & 0xFFFFFFFF
+    tasks = torch.where(tasks >= (1 << 31), tasks - (1 << 32), tasks).to(torch.int32)
+    grp_task_ptr = torch.cat([torch.zeros(1, dtype=i64, device=d),
+                              torch.cumsum(torch.bincount(gg, minlength=G), 0)]).to(torch.int32)
+    return grp_task_ptr.cpu().numpy(), np.ascontiguousarray(tasks.cpu().numpy())

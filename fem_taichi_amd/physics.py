@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .groups import DEFAULT_BRICK, build_row_groups, build_tasks, node_group_ids
+from .groups import DEFAULT_BRICK, build_row_groups, build_tasks_torch, node_group_ids
 from .runtime import device, stream_ptr
 
 # Upper bound on the elements one row group may touch (LDS staging budget).
@@ -141,7 +141,8 @@ class Physics:
         gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=self.brick)
         groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, MAX_GROUP_ELEMS,
                                   co_node=self.CoNode)
-        tp, tasks = build_tasks(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr, ne_list, groups)
+        tp, tasks = build_tasks_torch(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr, ne_list,
+                                      groups, device=device())
         groups["grp_task_ptr"], groups["tasks"] = tp, tasks
         groups["grp_enodes"] = np.ascontiguousarray(self.elem_nodes[groups["grp_elems"]].reshape(-1))
         groups["max_group_tasks"] = int(np.diff(tp).max()) if len(tp) > 1 else 0
@@ -164,8 +165,8 @@ class Physics:
                                  rows=KN_MORTON_ROWS)
             groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, KN_MAX_GROUP_ELEMS,
                                       co_node=self.CoNode)
-            tp, tasks = build_tasks(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr, ne_list,
-                                    groups)
+            tp, tasks = build_tasks_torch(self.Num_Node, self.elem_nodes, self.SmNode, self.ReNode, ne_ptr,
+                                          ne_list, groups, device=device())
             groups["grp_task_ptr"], groups["tasks"] = tp, tasks
             groups["grp_enodes"] = np.ascontiguousarray(self.elem_nodes[groups["grp_elems"]].reshape(-1))
             groups["max_group_tasks"] = int(np.diff(tp).max()) if len(tp) > 1 else 0

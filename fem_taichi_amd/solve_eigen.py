@@ -225,13 +225,13 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
         v = ops.solve(ops.m_apply(u), inner_tol)
         Tii = gram(u, ops.m_apply(v))
         T[i * b:(i + 1) * b, i * b:(i + 1) * b] = Tii
-        v = v - u @ Tii
+        v = torch.addmm(v, u, Tii, alpha=-1.0)
         if i >= steps - 1:
             break
         k = i * b + b
         Qk = Q[:, :k]
-        v = v - Qk @ gram(Qk, ops.m_apply(v))
-        v = v - Qk @ gram(Qk, ops.m_apply(v))
+        v = torch.addmm(v, Qk, gram(Qk, ops.m_apply(v)), alpha=-1.0)
+        v = torch.addmm(v, Qk, gram(Qk, ops.m_apply(v)), alpha=-1.0)
         t = _chol_upper(gram(v, ops.m_apply(v)))
         T[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = t
         T[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = t.T

@@ -203,3 +203,26 @@ def test_spmm_multi(setup, nrhs):
     _lib.call("fem_mass_spmm", model.dev.ref, model.M_blocks().data_ptr(), Xd.data_ptr(), Y.data_ptr(), nrhs,
               stream_ptr())
     assert rel(Y.cpu().numpy(), M @ X) < 1e-14
+
+
+def test_tasks_on_device_equal_host(gpu):
+    """The assembly task lists built on the GPU (groups.build_tasks_torch, what
+    Physics uses) equal the host numpy build on a 24x20x18 jittered slab."""
+    from fem_taichi_amd import Physics, groups as Gr
+    g = mesh.structured_block(24, 20, 18, 1.0, 0.1, 3, E_sigma=0.3)
+    phys = Physics.from_grid(g)
+    pp = O.pre_process(g.Node, g.Elem)
+    en = O.element_nodes(g.Elem, pp["Nind"])
+    N = g.Node.shape[1]
+    e = np.repeat(np.arange(len(en)), 8)
+    j = np.tile(np.arange(8), len(en))
+    node = en.reshape(-1)
+    o = np.lexsort((e * 8 + j, node))
+    ne_ptr = np.concatenate([[0], np.cumsum(np.bincount(node, minlength=N))])
+    ne_list = (e * 8 + j)[o]
+    gid = Gr.node_group_ids(N, node_dims=(25, 21, 19))
+    groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 48, co_node=pp["CoNode"])
+    a = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    b = Gr.build_tasks_torch(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups, device="cuda")
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert phys.dev.groups_info["num_groups"] == groups["num_groups"]

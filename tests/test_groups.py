@@ -237,3 +237,42 @@ def test_tasks_crowded_wide_butterfly():
     assert np.all(hits == 1)
     Kref = O.assemble_K_bsr(Ke, en, pp["SmNode"], pp["ReNode"])
     assert np.linalg.norm(Kt.reshape(-1) - Kref.reshape(-1)) / np.linalg.norm(Kref) < 1e-14
+
+
+@pytest.mark.parametrize("name,kw,brick", CASES, ids=[c[0] for c in CASES])
+def test_tasks_torch_equal_numpy(name, kw, brick):
+    """build_tasks_torch (the device form used on GPU machines, run here on
+    the CPU) gives the numpy build_tasks' lists exactly."""
+    g = mesh.structured_block(*kw["dims"], 1.0, kw["jitter"], kw["seed"], E_sigma=0.3)
+    Node, Elem = (mesh.permute_ids(g, seed=kw["seed"]) if name == "permuted" else (g.Node, g.Elem))
+    pp = O.pre_process(Node, Elem)
+    en = O.element_nodes(Elem, pp["Nind"])
+    N = Node.shape[1]
+    ne_ptr, ne_list = incidences(en, N)
+    if brick:
+        nx, ny, nz = kw["dims"]
+        gid = Gr.node_group_ids(N, node_dims=(nx + 1, ny + 1, nz + 1))
+    else:
+        gid = Gr.node_group_ids(N, xyz=np.asarray(Node[-3:], dtype=np.float64).T)
+    groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 48, co_node=pp["CoNode"])
+    a = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    b = Gr.build_tasks_torch(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+    assert np.array_equal(a[0], b[0]) and a[1].dtype == b[1].dtype and np.array_equal(a[1], b[1])
+
+
+def test_tasks_torch_equal_numpy_crowded_and_mesh2():
+    from fixtures import load_mesh2
+    g, elem, E = crowded_mesh()
+    meshes = [(g.Node, elem)]
+    Node, Elem, _, _ = load_mesh2()
+    meshes.append((Node, Elem))
+    for Node, Elem in meshes:
+        pp = O.pre_process(Node, Elem)
+        en = O.element_nodes(Elem, pp["Nind"])
+        N = Node.shape[1]
+        ne_ptr, ne_list = incidences(en, N)
+        gid = Gr.node_group_ids(N, xyz=np.asarray(Node[-3:], dtype=np.float64).T)
+        groups = Gr.build_row_groups(N, len(en), ne_ptr, ne_list, gid, 64, co_node=pp["CoNode"])
+        a = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+        b = Gr.build_tasks_torch(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
