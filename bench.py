@@ -218,8 +218,8 @@ def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the assembly and solvers would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the assembly would shard by node "
+              "slabs with one halo element layer, the solvers by row blocks with halo exchanges).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

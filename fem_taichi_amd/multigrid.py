@@ -2,7 +2,7 @@
 
 No counterpart in the reference, whose static solve is a Pardiso
 factorisation (SOLVE_static.py:31-38); this is one of the GPU forms that
-replace it, judged like the others by the FP64 residual ||F - K U|| / ||F|| on
+replace it, measured like the others by the FP64 residual ||F - K U|| / ||F|| on
 the penalised K.
 
 Hierarchy (symbolic, once per mesh, cached on the ``Physics``): the node grid
