@@ -303,7 +303,7 @@ gemm_tn_sum_kernel(const double* __restrict__ part, int pq, int nblk, double* __
 // column), folded over subgroups in order, and a one-work-group kernel folds
 // the partials in work-group order: deterministic, no atomics.
 constexpr int kFT = 256;
-constexpr int kFcgGrid = 1024;
+constexpr int kFcgGrid = 4096;   // 4,096 x 256 threads: enough loads in flight to stream HBM
 // per-column scalars after the partials (doubles)
 enum { kScBeta = 0, kScAlpha = 1, kScPqPrev = 2, kScCount = 4 };
 
@@ -347,6 +347,7 @@ fcg_dot_kernel(int64_t n, int nb, const TA* __restrict__ A, const double* __rest
   const FcgMap f = fcg_map(n, nb);
   double acc = 0.0;
   if (f.on)
+#pragma unroll 4
     for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
       const double a = (double)A[i * nb + f.c] * B[i * nb + f.c];
       acc += mask ? a * mask[i] : a;
@@ -362,6 +363,7 @@ fcg_dot2_kernel(int64_t n, int nb, const double* __restrict__ P, const double* _
   const FcgMap f = fcg_map(n, nb);
   double pq = 0.0, pr = 0.0;
   if (f.on)
+#pragma unroll 4
     for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
       const int64_t e = i * nb + f.c;
       const double p = P[e];
@@ -431,6 +433,7 @@ fcg_dir_kernel(int64_t n, int nb, const TZ* __restrict__ Z, const double* __rest
   const FcgMap f = fcg_map(n, nb);
   if (!f.on) return;
   const double beta = first ? 0.0 : sc[kScBeta * nb + f.c];
+#pragma unroll 4
   for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
     const int64_t e = i * nb + f.c;
     double z = (double)Z[e];
@@ -449,6 +452,7 @@ fcg_update_kernel(int64_t n, int nb, const double* __restrict__ P, const double*
   double rr = 0.0;
   if (f.on) {
     const double alpha = sc[kScAlpha * nb + f.c];
+#pragma unroll 4
     for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
       const int64_t e = i * nb + f.c;
       X[e] += alpha * P[e];
