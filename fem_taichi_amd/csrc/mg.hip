@@ -196,6 +196,7 @@ smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __r
     const int64_t sm = m.sm_node[i];
     const int co = m.co_node[i];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll 4   // more gathers in flight (1.55 -> 1.31 ms at 8 columns on 2.3M nodes)
     for (int b = 0; b < co; ++b) {
       const float* __restrict__ v = vals + 9 * (sm + b);
       const int64_t xc = 3 * (int64_t)m.re_node[sm + b] * nb + c;
@@ -220,98 +221,6 @@ smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __r
     const float d = w * (D[3 * p] * r0 + D[3 * p + 1] * r1 + D[3 * p + 2] * r2);
     OUT[o + p * nb] = MODE == 1 ? X[o + p * nb] + d : d;
   }
-}
-
-// smooth_multi_kernel with CW adjacent columns per thread: each K block is
-// loaded once for the CW columns and the X, B, OUT rows move as CW-float
-// vectors (nb % CW == 0, 4*CW-byte aligned buffers; CW = 2 is used).  Same arithmetic per
-// column as smooth_multi_kernel.
-template <int CW>
-struct FVec;
-template <>
-struct FVec<2> { using T = float2; };
-
-template <int CW>
-__device__ __forceinline__ void vload(const float* p, float (&v)[CW]) {
-  const typename FVec<CW>::T x = *reinterpret_cast<const typename FVec<CW>::T*>(p);
-  const float* f = reinterpret_cast<const float*>(&x);
-#pragma unroll
-  for (int k = 0; k < CW; ++k) v[k] = f[k];
-}
-template <int CW>
-__device__ __forceinline__ void vstore(float* p, const float (&v)[CW]) {
-  typename FVec<CW>::T x;
-  float* f = reinterpret_cast<float*>(&x);
-#pragma unroll
-  for (int k = 0; k < CW; ++k) f[k] = v[k];
-  *reinterpret_cast<typename FVec<CW>::T*>(p) = x;
-}
-
-template <int MODE, int CW>
-__global__ void __launch_bounds__(kThreads)
-smooth_multi_cw_kernel(fem_mesh m, const float* __restrict__ vals, const float* __restrict__ dinv, float w,
-                       const float* __restrict__ X, const float* __restrict__ B, float* __restrict__ OUT, int nb) {
-  const int ng = nb / CW;
-  const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  if (t >= m.num_node * ng) return;
-  const int64_t i = t / ng;
-  const int c = CW * (int)(t - i * ng);
-  const int64_t o = 3 * i * nb + c;
-  float r[3][CW];
-#pragma unroll
-  for (int p = 0; p < 3; ++p) vload<CW>(B + o + p * nb, r[p]);
-  if (MODE != 2) {
-    const int64_t sm = m.sm_node[i];
-    const int co = m.co_node[i];
-    float a[3][CW];
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int k = 0; k < CW; ++k) a[p][k] = 0.f;
-    for (int b = 0; b < co; ++b) {
-      const float* __restrict__ v = vals + 9 * (sm + b);
-      const int64_t xc = 3 * (int64_t)m.re_node[sm + b] * nb + c;
-      float x[3][CW];
-#pragma unroll
-      for (int q = 0; q < 3; ++q) vload<CW>(X + xc + q * nb, x[q]);
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-#pragma unroll
-        for (int k = 0; k < CW; ++k) a[p][k] += v[3 * p] * x[0][k] + v[3 * p + 1] * x[1][k] + v[3 * p + 2] * x[2][k];
-    }
-#pragma unroll
-    for (int p = 0; p < 3; ++p)
-#pragma unroll
-      for (int k = 0; k < CW; ++k) r[p][k] -= a[p][k];
-  }
-  if (MODE == 0) {
-#pragma unroll
-    for (int p = 0; p < 3; ++p) vstore<CW>(OUT + o + p * nb, r[p]);
-    return;
-  }
-  const float* __restrict__ D = dinv + 9 * i;
-#pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    float d[CW];
-    if (MODE == 1) vload<CW>(X + o + p * nb, d);
-#pragma unroll
-    for (int k = 0; k < CW; ++k) {
-      const float u = w * (D[3 * p] * r[0][k] + D[3 * p + 1] * r[1][k] + D[3 * p + 2] * r[2][k]);
-      d[k] = MODE == 1 ? d[k] + u : u;
-    }
-    vstore<CW>(OUT + o + p * nb, d);
-  }
-}
-
-// Columns per thread of the multi-column smoother for nb columns.  Measured
-// on the 2.3M-node blade (residual mode): 8 columns 1.55 / 1.64 / 2.34 ms and
-// 16 columns 2.13 / 2.04 / 2.11 ms for 1 / 2 / 4 columns per thread; the X
-// gathers, not the K blocks, set the time.
-int smooth_cw(int nb, const void* X, const void* B, const void* OUT) {
-  const uintptr_t al = reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(B) |
-                       reinterpret_cast<uintptr_t>(OUT);
-  if (nb >= 16 && nb % 2 == 0 && al % 8 == 0) return 2;
-  return 1;
 }
 
 // Y = A X for the coarsest level's dense inverse, X and Y n x nb row-major.
@@ -445,21 +354,16 @@ extern "C" int fem_mg_smooth_multi(const fem_mesh* mesh, const float* vals, cons
   if (T == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
   const int nb = (int)nrhs;
-  const int cw = smooth_cw(nb, X ? X : B, B, OUT);
-  const int64_t T2 = mesh->num_node * (nb / cw);
-#define FEM_SMC(MD, CWV) \
-  smooth_multi_cw_kernel<MD, CWV><<<ceil_div(T2, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb)
-  if (cw == 2) {
-    if (mode == 0) FEM_SMC(0, 2);
-    else if (mode == 1) FEM_SMC(1, 2);
-    else FEM_SMC(2, 2);
-  } else if (mode == 0)
+  // One column per thread: measured on the 2.3M-node blade against two and
+  // four columns per thread (residual mode, unrolled block loop), 8 columns
+  // take 1.31 ms and 16 columns 1.94 ms (2 per thread: 1.98 ms); the X
+  // gathers, not the K blocks, set the time.
+  if (mode == 0)
     smooth_multi_kernel<0><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
   else if (mode == 1)
     smooth_multi_kernel<1><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
   else
     smooth_multi_kernel<2><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
-#undef FEM_SMC
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

@@ -92,8 +92,7 @@ def test_smoother_modes(gpu, layout):
 
 @pytest.mark.parametrize("nb", [3, 8, 16, 18])
 def test_smoother_multi_matches_single(gpu, nb):
-    """fem_mg_smooth_multi on nb columns (1 or, from 16 even columns on, 2
-    columns per thread) equals fem_mg_smooth per column."""
+    """fem_mg_smooth_multi on nb columns equals fem_mg_smooth per column."""
     import torch
     from fem_taichi_amd import Model, Physics, _lib, solvers
     from fem_taichi_amd.runtime import stream_ptr
