@@ -137,10 +137,11 @@ class MultigridBlockOps:
     improving); one that fails otherwise within ``maxiter`` is finished by
     FP64 block-Jacobi PCG."""
 
-    def __init__(self, model, m_apply, min_nodes=1000, k_vals=None):
+    def __init__(self, model, m_apply, min_nodes=1000, k_vals=None, patience=2):
         from . import multigrid, solvers
         from .learned import free_dof_mask
         self.model = model
+        self.patience = patience      # true-residual checks without halving before a column is at its floor
         self.K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64") if k_vals is None else k_vals)
         self.pre = multigrid.MultigridPreconditioner(model, multigrid.hierarchy_for(model, min_nodes))
         self.dinv64, _ = self.K64.block_jacobi()
@@ -155,7 +156,8 @@ class MultigridBlockOps:
         if B.shape[1] > MAX_BLOCK:
             return _in_pieces(self, B, tol, maxiter)
         X, info = multigrid.solve_block_fcg_mg(self.model, self.K64, B.contiguous(), self.pre, tol=tol,
-                                               maxiter=maxiter, free=self.free, dinv64=self.dinv64)
+                                               maxiter=maxiter, free=self.free, dinv64=self.dinv64,
+                                               patience=self.patience)
         total = info["iterations"] * B.shape[1]
         for j in np.nonzero(~(info["converged"] | info["stagnated"]))[0]:
             x, it2, _ = solvers.pcg(self.K64, B[:, j].contiguous(), self.dinv64, tol, 200000,
