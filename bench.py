@@ -144,7 +144,11 @@ def time_solves(mesh_spec, all_forms=False):
     run("mg_fp32_vcycle_fp64_fcg", lambda: multigrid.solve_mg(model, K64, F, tol=1e-8)[1],
         lambda info: {"setup_seconds": info.history[0]["setup_seconds"]})
     run("learned_mg", lambda: solve_learned(model, K64, F, tol=1e-8, base="mg")[1],
-        lambda info: {"fell_back": bool(info.history[0]["fell_back"])})
+        lambda info: {"fell_back": bool(info.history[0]["fell_back"]),
+                      "net_applications": info.history[0]["net_applications"]})
+    run("learned_mg_net_every_iteration", lambda: solve_learned(model, K64, F, tol=1e-8, base="mg", min_gain=None)[1],
+        lambda info: {"fell_back": bool(info.history[0]["fell_back"]),
+                      "net_applications": info.history[0]["net_applications"]})
     if all_forms:
         run("mixed_fp32_pcg_fp64_refine", lambda: solvers.solve_mixed(K64, K32, F, tol=1e-8)[1],
             lambda info: {"outer": info.outer})

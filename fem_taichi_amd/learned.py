@@ -205,7 +205,8 @@ class LearnedPreconditioner:
     minimises the K-norm of the error along c (so the learned term can only
     lower it)."""
 
-    def __init__(self, net: MeshConvNet, node_dims, efeat, dinv64, free_mask=None, use_net=True, base=None, K=None):
+    def __init__(self, net: MeshConvNet, node_dims, efeat, dinv64, free_mask=None, use_net=True, base=None, K=None,
+                 min_gain=None, gain_patience=3):
         self.net = net
         self.grid = ConvWork(*node_dims)
         self.efeat = efeat
@@ -218,6 +219,11 @@ class LearnedPreconditioner:
             raise ValueError("a base preconditioner needs the operator K for its residual")
         self.applications = 0
         self.taus = []
+        self.gains = []
+        self.min_gain = min_gain
+        self.gain_patience = gain_patience
+        self.low = 0
+        self.net_off = False
 
     def features(self, r):
         P = self.grid.P
@@ -238,13 +244,24 @@ class LearnedPreconditioner:
                 return z
             return z + self.correction(r)
         z = self.base(r)
-        if not self.use_net:
+        if not self.use_net or self.net_off:
             return z
         ra = r - self.K.matvec(z)
         c = self.correction(ra)
         kc = self.K.matvec(c)
-        tau = torch.nan_to_num(torch.dot(c, ra) / torch.dot(c, kc), nan=0.0, posinf=0.0, neginf=0.0)
+        cra = torch.dot(c, ra)
+        tau = torch.nan_to_num(cra / torch.dot(c, kc), nan=0.0, posinf=0.0, neginf=0.0)
         self.taus.append(tau)
+        if self.min_gain is not None:
+            # the error energy the correction removes, tau (c.r_a), against the
+            # energy r.z of the V-cycle's own step; after ``gain_patience``
+            # applications in a row below ``min_gain`` the network is switched
+            # off for the rest of the solve (the V-cycle alone from then on)
+            gain = float(torch.nan_to_num(tau * cra / torch.dot(r, z), nan=0.0))
+            self.gains.append(gain)
+            self.low = self.low + 1 if gain < self.min_gain else 0
+            if self.low >= self.gain_patience:
+                self.net_off = True
         return z + tau * c
 
     def correction(self, r, keep=False):
@@ -353,13 +370,16 @@ def solve_fcg(K64, K32, F, precond, tol=1e-8, maxiter=100000, check_every=8, dro
 
 
 def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=100000, use_net=True,
-                  matrix="f32", base="mg", **kw):
+                  matrix="f32", base="mg", min_gain=0.02, gain_patience=3, **kw):
     """Static solve with the learned preconditioner (structured meshes).
 
     base "mg": flexible CG (multigrid.solve_fcg_mg) preconditioned by the FP32
     multigrid V-cycle plus the learned correction of the residual it leaves
     (weights of the "mg" role); when that does not reach ``tol`` in
     ``maxiter`` iterations, FP64 block-Jacobi PCG finishes from its iterate.
+    The correction is dropped for the rest of the solve once it has removed
+    less than ``min_gain`` of the V-cycle step's energy in ``gain_patience``
+    applications in a row (min_gain None: never).
     base "bj": truncated flexible CG (solve_fcg) with block-Jacobi plus the
     learned correction (weights of the "bj" role); ``matrix`` "f32" streams
     the FP32 copy of K in that iteration, "f64" the FP64 one."""
@@ -383,11 +403,13 @@ def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=10000
         t0 = time.perf_counter()                # the symbolic hierarchy is per-mesh preprocessing
         mg = multigrid.MultigridPreconditioner(model, hier)
         pre = LearnedPreconditioner(net, phys.node_dims, efeat, dinv64, free_mask=free_dof_mask(model),
-                                    use_net=use_net, base=mg, K=K64)
+                                    use_net=use_net, base=mg, K=K64, min_gain=min_gain, gain_patience=gain_patience)
         x, info = multigrid.solve_fcg_mg(model, K64, F, pre, tol=tol, maxiter=min(maxiter, 1000), **kw)
         info["fell_back"] = False
         info["applications"] = pre.applications
+        info["net_applications"] = len(pre.taus)
         info["taus"] = [float(v) for v in pre.taus[:64]]
+        info["gains"] = [round(v, 5) for v in pre.gains[:64]]
         if not info["converged"]:
             info["fell_back"] = True
             x, it2, _ = solvers.pcg(K64, F, dinv64, tol, maxiter, x0=x.clone())

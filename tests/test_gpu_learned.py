@@ -87,7 +87,7 @@ def test_learned_mg_ablation(trained_mg):
     K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
     runs = {}
     for use_net in (True, False):
-        x, info = L.solve_learned(model, K64, F, tol=1e-8, net=net, base="mg", use_net=use_net)
+        x, info = L.solve_learned(model, K64, F, tol=1e-8, net=net, base="mg", use_net=use_net, min_gain=None)
         r = F - K64.matvec(x)
         assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(F)) < 1e-8
         assert not info.history[0]["fell_back"]
@@ -111,3 +111,20 @@ def test_shipped_mg_weights_solve_slab(gpu):
     r = F - K64.matvec(x)
     assert float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(F)) < 1e-8
     assert info.iterations < 40
+
+
+def test_learned_mg_switches_network_off(gpu):
+    """The learned form drops the network after gain_patience applications in
+    a row whose energy gain is below min_gain (1.0 here, so always): exactly
+    two network applications, then the V-cycle alone, to the FP64 target."""
+    from fem_taichi_amd import Model, Physics, solvers
+    from fem_taichi_amd import learned as L
+    g = mesh.twisted_blade(40, 8, 8)
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=g.Omegas)
+    F = model.F_device()
+    K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+    x, info = L.solve_learned(model, K64, F, tol=1e-8, base="mg", min_gain=1.0, gain_patience=2)
+    h = info.history[0]
+    assert h["net_applications"] == 2 and len(h["gains"]) == 2 and not h["fell_back"]
+    assert float(torch.linalg.vector_norm(F - K64.matvec(x)) / torch.linalg.vector_norm(F)) < 1e-8
