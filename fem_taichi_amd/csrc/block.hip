@@ -374,15 +374,21 @@ fcg_dot2_kernel(int64_t n, int nb, const double* __restrict__ P, const double* _
   fcg_block_fold(pr, f, nb, red, part_pr + (int64_t)blockIdx.x * nb);
 }
 
-// Column totals of nblk partials, in work-group order within each of the
-// G = 256 / nb strided subsets, then over the subsets in order.
+// Column totals of nblk partials by one 1024-thread work-group: thread
+// (subset s, column c) sums partials s, s + G, ... (G = min(1024 / nb, 256)
+// subsets, up to eight loads in flight), then thread c sums the subsets in
+// order.
+constexpr int kFoldT = 1024;
+
 __device__ __forceinline__ double fcg_fold_partials(const double* __restrict__ part, int nblk, int nb, double* red,
                                                     int* col) {
-  const int G = kFT / nb;
+  const int G = kFoldT / nb < 256 ? kFoldT / nb : 256;
   const int c = (int)threadIdx.x % nb, s = (int)threadIdx.x / nb;
   double v = 0.0;
-  if (s < G)
+  if (s < G) {
+#pragma unroll 8
     for (int g = s; g < nblk; g += G) v += part[(int64_t)g * nb + c];
+  }
   __syncthreads();
   red[threadIdx.x] = s < G ? v : 0.0;
   __syncthreads();
@@ -394,19 +400,19 @@ __device__ __forceinline__ double fcg_fold_partials(const double* __restrict__ p
 }
 
 // beta_c = (sum mask Z Qp)_c / pq_prev_c
-__global__ void __launch_bounds__(kFT)
+__global__ void __launch_bounds__(kFoldT)
 fcg_beta_kernel(const double* __restrict__ part, int nblk, int nb, double* __restrict__ sc) {
-  __shared__ double red[kFT];
+  __shared__ double red[kFoldT];
   int c;
   const double zq = fcg_fold_partials(part, nblk, nb, red, &c);
   if (c < nb) sc[kScBeta * nb + c] = zq / sc[kScPqPrev * nb + c];
 }
 
 // alpha_c = pr / pq on active columns (0 on frozen ones); pq kept for beta
-__global__ void __launch_bounds__(kFT)
+__global__ void __launch_bounds__(kFoldT)
 fcg_alpha_kernel(const double* __restrict__ part_pq, const double* __restrict__ part_pr, int nblk, int nb,
                  const int32_t* __restrict__ active, double* __restrict__ sc) {
-  __shared__ double red[kFT];
+  __shared__ double red[kFoldT];
   int c;
   const double pq = fcg_fold_partials(part_pq, nblk, nb, red, &c);
   const double pr = fcg_fold_partials(part_pr, nblk, nb, red, &c);
@@ -417,9 +423,9 @@ fcg_alpha_kernel(const double* __restrict__ part_pq, const double* __restrict__ 
   }
 }
 
-__global__ void __launch_bounds__(kFT)
+__global__ void __launch_bounds__(kFoldT)
 fcg_sum_kernel(const double* __restrict__ part, int nblk, int nb, double* __restrict__ out) {
-  __shared__ double red[kFT];
+  __shared__ double red[kFoldT];
   int c;
   const double v = fcg_fold_partials(part, nblk, nb, red, &c);
   if (c < nb) out[c] = v;
@@ -579,7 +585,7 @@ extern "C" int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dty
     if (f32) fcg_dot_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, Qp, mask, part);
     else fcg_dot_kernel<double><<<grid, kFT, 0, st>>>(n, nbi, Zd, Qp, mask, part);
     FEM_LAUNCH_CHECK();
-    fcg_beta_kernel<<<1, kFT, 0, st>>>(part, grid, nbi, sc);
+    fcg_beta_kernel<<<1, kFoldT, 0, st>>>(part, grid, nbi, sc);
     FEM_LAUNCH_CHECK();
   }
   if (f32) fcg_dir_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, mask, sc, first ? 1 : 0, P);
@@ -604,11 +610,11 @@ extern "C" int fem_fcg_update(int64_t n, int64_t nb, const double* P, const doub
   if (n > 0) FEM_REQUIRE(P && Q && X && R, FEM_E_INVALID, "null pointer");
   fcg_dot2_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, R, part, part2);
   FEM_LAUNCH_CHECK();
-  fcg_alpha_kernel<<<1, kFT, 0, st>>>(part, part2, grid, nbi, active, sc);
+  fcg_alpha_kernel<<<1, kFoldT, 0, st>>>(part, part2, grid, nbi, active, sc);
   FEM_LAUNCH_CHECK();
   fcg_update_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, mask, sc, X, R, R32, part3);
   FEM_LAUNCH_CHECK();
-  fcg_sum_kernel<<<1, kFT, 0, st>>>(part3, grid, nbi, rr);
+  fcg_sum_kernel<<<1, kFoldT, 0, st>>>(part3, grid, nbi, rr);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

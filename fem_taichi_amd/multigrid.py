@@ -471,6 +471,16 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
                "trace": trace}
 
 
+def _colnorms(A):
+    """Column 2-norms of a tall (n, k) FP64 device block, as numpy: the
+    diagonal of A^T A from the deterministic row-split product (a library
+    reduction along the rows of a narrow row-major block is several times
+    slower)."""
+    from .solve_eigen import gram
+    A = A.contiguous()
+    return np.sqrt(np.maximum(torch.diagonal(gram(A, A)).cpu().numpy(), 0.0))
+
+
 def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
                        drop=1e-2):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
@@ -506,14 +516,14 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
         return Y
 
     B = B.contiguous()
-    bn = torch.linalg.vector_norm(B, dim=0).cpu().numpy()
+    bn = _colnorms(B)
     X = torch.zeros_like(B)
     R = B * freec
     R32 = R.float()
     active = bn > 0
     converged = ~active
     stagnated = np.zeros(nb, dtype=bool)
-    anchor = torch.linalg.vector_norm(R, dim=0).cpu().numpy()
+    anchor = _colnorms(R)
     best = np.full(nb, np.inf)
     stalls = np.zeros(nb, dtype=int)
     act_d = torch.as_tensor(active.astype(np.int32), device=B.device)
@@ -541,14 +551,14 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
             tn = np.zeros(nb)
             if 2 * len(cols) > nb:      # most columns: one product of all of them
                 Rt = (B - spmm(X)) * freec
-                tn[cols] = torch.linalg.vector_norm(Rt, dim=0).cpu().numpy()[cols]
+                tn[cols] = _colnorms(Rt)[cols]
                 sel = torch.as_tensor(need, device=B.device)
                 R.copy_(torch.where(sel[None, :], Rt, R))
                 R32.copy_(R)
             else:
                 ci = torch.as_tensor(cols, device=B.device)
                 Rt = (B[:, ci] - spmm_cols(X[:, ci].contiguous())) * freec
-                tn[cols] = torch.linalg.vector_norm(Rt, dim=0).cpu().numpy()
+                tn[cols] = _colnorms(Rt)
                 R[:, ci] = Rt
                 R32[:, ci] = Rt.float()
             for c in cols:
@@ -567,7 +577,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     # the penalised DOFs from their own rows
     Rp = (B - spmm(X)) * fixed[:, None]
     X.add_(torch.bmm(d3, Rp.view(-1, 3, nb)).view(n, nb) * fixed[:, None])
-    res = torch.linalg.vector_norm(B - spmm(X), dim=0).cpu().numpy() / np.where(bn > 0, bn, 1.0)
+    res = _colnorms(B - spmm(X)) / np.where(bn > 0, bn, 1.0)
     converged = converged & (res < tol)
     return X, {"iterations": it, "rel_residual": res, "converged": converged, "stagnated": stagnated}
 
