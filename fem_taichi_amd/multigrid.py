@@ -508,12 +508,8 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
         return Y
 
     def spmm_cols(X):
-        """K X for a contiguous (n, k) block of any width k."""
-        if X.shape[1] == 1:
-            return K64.matvec(X.view(-1)).view(-1, 1)
-        Y = torch.empty_like(X)
-        _lib.call("fem_bsr_spmm", K64.mesh.ref, K64.vals.data_ptr(), X.data_ptr(), Y.data_ptr(), X.shape[1], st)
-        return Y
+        """K X, column by column, for a narrow (n, k) block (k <= 2)."""
+        return torch.stack([K64.matvec(X[:, j].contiguous()) for j in range(X.shape[1])], dim=1)
 
     B = B.contiguous()
     bn = _colnorms(B)
@@ -545,11 +541,11 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
             break
         need = active & ((rf < drop * anchor) | (rf < tol * bn))
         if need.any():
-            # true residuals of the columns that asked for one (a product of
-            # only those columns: most steps replace one or two of nb)
+            # true residuals of the columns that asked for one (one or two
+            # of them by single-vector products)
             cols = np.nonzero(need)[0]
             tn = np.zeros(nb)
-            if 2 * len(cols) > nb:      # most columns: one product of all of them
+            if len(cols) > 2:           # one product of all columns costs about as much as two SpMVs
                 Rt = (B - spmm(X)) * freec
                 tn[cols] = _colnorms(Rt)[cols]
                 sel = torch.as_tensor(need, device=B.device)
