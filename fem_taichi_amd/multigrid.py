@@ -362,8 +362,17 @@ class FcgSteps:
         self.work = torch.empty(self.wb, dtype=torch.uint8, device=dev)
         self.rr = torch.empty(self.nb, dtype=torch.float64, device=dev)
 
+    def _check(self, *ts):
+        for t in ts:
+            if t is not None and (t.numel() != self.n * self.nb or not t.is_contiguous()):
+                raise ValueError(f"FCG vectors must be contiguous with {self.n} x {self.nb} entries")
+
     def direction(self, Z, mask, Qp, P, first):
         """P = mask Z - beta P (first: P = mask Z); Z FP32 or FP64."""
+        if Z.dtype not in (torch.float32, torch.float64) or P.dtype != torch.float64:
+            raise ValueError("FCG direction needs an FP32 or FP64 Z and an FP64 P")
+        Z = Z.contiguous()
+        self._check(Z, Qp, P)
         zdt = _lib.DTYPE_F32 if Z.dtype == torch.float32 else _lib.DTYPE_F64
         _lib.call("fem_fcg_direction", self.n, self.nb, Z.data_ptr(), zdt, _ptr(mask), _ptr(Qp), P.data_ptr(),
                   1 if first else 0, self.work.data_ptr(), self.wb, stream_ptr())
@@ -371,6 +380,7 @@ class FcgSteps:
     def update(self, P, Q, mask, active, X, R, R32):
         """alpha = p.r / p.q (0 on inactive columns), X += alpha P,
         R -= alpha mask Q, R32 = (float) R; returns rr = |R|^2 per column."""
+        self._check(P, Q, X, R, R32)
         _lib.call("fem_fcg_update", self.n, self.nb, P.data_ptr(), Q.data_ptr(), _ptr(mask), _ptr(active),
                   X.data_ptr(), R.data_ptr(), _ptr(R32), self.rr.data_ptr(), self.work.data_ptr(), self.wb,
                   stream_ptr())
