@@ -28,6 +28,14 @@ from .runtime import device, stream_ptr
 PENALTY = 1e20   # FEM_model_csr.py:241
 
 
+def _to_host(t):
+    """A device tensor as a numpy array, copied through a pinned buffer (about
+    three times the rate of a copy into pageable memory)."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t)
+    return h.numpy()
+
+
 def stage_bounds(Bounds):
     """FEM_model_csr.py:64-73: (4, nb) [ID-1, c0, c1, c2]; with no bounds the
     reference stages one zero column, which fixes node Nind[0]."""
@@ -94,6 +102,7 @@ class Model:
         np.add.at(load, nind[fid], forces[1:4].T)
         self.nodal_load = torch.as_tensor(load.reshape(-1)).to(d)
         self._cache = {}
+        self._csr_k_host = None
 
     # ---------------------------------------------------------------- device forms
     def _young_ptr(self):
@@ -182,7 +191,7 @@ class Model:
         kt, km = self.assemble_KN("csr")
         row, col = self._csr_index_K()
         n = self.Num_Free
-        return tuple(sp.csr_matrix((v.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64) for v in (kt, km))
+        return tuple(sp.csr_matrix((_to_host(v), col, row), shape=(n, n), dtype=np.float64) for v in (kt, km))
 
     # ---- prestressed stiffness for modal analysis (no counterpart in the
     #      reference; Readme.md:189 lists stress stiffening and spin softening
@@ -218,22 +227,25 @@ class Model:
         k = self.assemble_KS(displace, omega, "csr")
         row, col = self._csr_index_K()
         n = self.Num_Free
-        return sp.csr_matrix((k.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64)
+        return sp.csr_matrix((_to_host(k), col, row), shape=(n, n), dtype=np.float64)
 
     def _csr_index_K(self):
-        """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173), host copies."""
-        d = device()
-        row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
-        col = torch.empty(max(self.nnz_K, 1), dtype=torch.int32, device=d)
-        _lib.call("fem_csr_index_K", self.dev.ref, row.data_ptr(), col.data_ptr(), stream_ptr())
-        return row.cpu().numpy(), col[:self.nnz_K].cpu().numpy()
+        """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173), host copies,
+        built once per Model (they depend on the pattern only) and shared by the matrices returned."""
+        if self._csr_k_host is None:
+            d = device()
+            row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
+            col = torch.empty(max(self.nnz_K, 1), dtype=torch.int32, device=d)
+            _lib.call("fem_csr_index_K", self.dev.ref, row.data_ptr(), col.data_ptr(), stream_ptr())
+            self._csr_k_host = (_to_host(row), _to_host(col[:self.nnz_K]))
+        return self._csr_k_host
 
     def Get_MatrixKL(self):
         """FEM_model_csr.py:118-127: penalised stiffness as scipy CSR (3N x 3N)."""
         vals = self.assemble_K("f64", "csr")
         row, col = self._csr_index_K()
         n = self.Num_Free
-        return sp.csr_matrix((vals.cpu().numpy(), col, row), shape=(n, n), dtype=np.float64)
+        return sp.csr_matrix((_to_host(vals), col, row), shape=(n, n), dtype=np.float64)
 
     Get_MatrixK = Get_MatrixKL
 
