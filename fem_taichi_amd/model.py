@@ -260,8 +260,8 @@ class Model:
         _lib.call("fem_csr_index_M", self.dev.ref, row.data_ptr(), col.data_ptr(), st)
         _lib.call("fem_csr_values_M", self.dev.ref, mb.data_ptr(), val.data_ptr(), st)
         n = self.Num_Free
-        return sp.csr_matrix((val[:self.nnz_M].cpu().numpy(), col[:self.nnz_M].cpu().numpy(),
-                              row.cpu().numpy()), shape=(n, n), dtype=np.float64)
+        return sp.csr_matrix((_to_host(val[:self.nnz_M]), _to_host(col[:self.nnz_M]), _to_host(row)),
+                             shape=(n, n), dtype=np.float64)
 
     def Get_VectorF(self):
         """FEM_model_csr.py:141-146: load vector in DOF order 3*i + p."""
