@@ -22,8 +22,10 @@ assemble_M_csr         FEM_model_csr.py:178-192                    brute-force d
 modify_K               FEM_model_csr.py:233-242 / bsr:248-259      exact 1e20 set (not add)
 vector_F               FEM_model_csr.py:141-146, 205-229           linearity, closed form
 get_stress             POST_process.py:72-99, 172-223, 277-321     uniform-strain patch test (D,B);
-                                                                   eigh; Q3/Q4 restated from
-                                                                   source text only
+                                                                   eigh; the Q3 weight order and
+                                                                   the Q4 von Mises term: parity
+                                                                   unpinned (restated from the
+                                                                   source text only)
 lanczos                SOLVE_eigen.py:45-89                        dense scipy eigh on small pencils,
                                                                    the SOLVE_eigen.py:101-117 pencil
 element_KN             FEM_model_bsr.py:476-598 (UnitK, BN, S)     U = 0 gives element_K; rigid
