@@ -161,6 +161,14 @@ def time_solves(mesh_spec, all_forms=False):
             out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual,
                         "fell_back": bool(info.history[0]["fell_back"])}
             print(f"# {key} {json.dumps(out[key])}", flush=True)
+    # per-iteration roofline of plain PCG: 76 nnzb + 364 N bytes per iteration
+    # (SpMV with its block and column index reads, 12 vector passes, the 3x3
+    # block-Jacobi inverse), counted once per iteration
+    pc = out["pcg_fp64_bj"]
+    it_bytes = 76 * model.nnzb + 364 * g.Num_Node
+    gbs = it_bytes * pc["iterations"] / pc["seconds"] / 1e9
+    out["pcg_iteration_roofline"] = {"alg_bytes_per_iteration": it_bytes, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                                     "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     base = out["pcg_fp64_bj"]["seconds"]
     out["speedup_vs_pcg_fp64_bj"] = {k: base / v["seconds"] for k, v in out.items()
                                      if isinstance(v, dict) and "seconds" in v and k != "pcg_fp64_bj"}
