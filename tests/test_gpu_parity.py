@@ -48,6 +48,7 @@ def crowded():
 
 def cases():
     return {
+        "single_hex": mesh.structured_block(1, 1, 1, 1.0, 0.0, 0),   # one group, one element, z = 0 face fixed
         "box_3x4x5": mesh.structured_block(3, 4, 5, 1.0, 0.0, 0),
         "jit_6x5x7_lognormal": mesh.structured_block(6, 5, 7, 1.0, 0.10, 1, E_sigma=0.3),
         "twisted_10x4x4": mesh.twisted_blade(10, 4, 4, seed=3),
@@ -226,3 +227,34 @@ def test_tasks_on_device_equal_host(gpu):
     b = Gr.build_tasks_torch(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups, device="cuda")
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     assert phys.dev.groups_info["num_groups"] == groups["num_groups"]
+
+
+def test_pattern_input_checks(gpu):
+    """The pattern builder's own checks turn bad connectivity into errors
+    through the C ABI (a node index outside [0, N): FEM_E_INVALID; a node
+    with more than 128 distinct neighbours: FEM_E_LIMIT), and an empty
+    element table gives an empty pattern."""
+    import torch
+    from fem_taichi_amd import _lib
+    from fem_taichi_amd.physics import gpu_pattern
+    fan = np.zeros((20, 8), np.int32)            # 20 hexes around node 0: 141 neighbours
+    fan[:, 1:] = 1 + np.arange(140).reshape(20, 7)
+    with pytest.raises(_lib.FemError, match="neighbour") as ei:
+        gpu_pattern(torch.as_tensor(fan).cuda(), 141)
+    assert ei.value.code == _lib.FEM_E_LIMIT
+    ok = gpu_pattern(torch.as_tensor(fan[:18]).cuda(), 127)   # 127 neighbours: within the limit
+    nbrs = [set() for _ in range(127)]
+    for row in fan[:18]:
+        for a in row:
+            nbrs[a].update(row.tolist())
+    assert ok["co_node"].cpu().tolist() == [len(s) for s in nbrs]
+    assert ok["re_node"][:127].cpu().tolist() == list(range(127))
+    bad = fan[:2].copy()
+    bad[1, 3] = 500
+    with pytest.raises(_lib.FemError, match="outside"):
+        gpu_pattern(torch.as_tensor(bad).cuda(), 141)
+    bad[1, 3] = -1
+    with pytest.raises(_lib.FemError, match="outside"):
+        gpu_pattern(torch.as_tensor(bad).cuda(), 141)
+    empty = gpu_pattern(torch.zeros((0, 8), dtype=torch.int32).cuda(), 5)
+    assert empty["co_node"].cpu().tolist() == [0] * 5 and int(empty["sm_node"][5].item()) == 0
