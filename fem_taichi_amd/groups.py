@@ -35,7 +35,8 @@ def node_group_ids(N, node_dims=None, xyz=None, brick=DEFAULT_BRICK, rows=DEFAUL
     """Group id of every node (groups numbered in processing order)."""
     if node_dims is not None:
         nxn, nyn, nzn = node_dims
-        assert nxn * nyn * nzn == N
+        if nxn * nyn * nzn != N:
+            raise ValueError(f"node_dims {tuple(node_dims)} do not multiply to the {N} nodes")
         bx, by, bz = brick
         idx = np.arange(N)
         i, j, k = idx % nxn, (idx // nxn) % nyn, idx // (nxn * nyn)

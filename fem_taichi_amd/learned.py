@@ -60,7 +60,8 @@ class MeshConvNet:
     """3 layers of 3x3x3 convolution: 4 -> 64 -> 64 -> 3 (ReLU between)."""
 
     def __init__(self, seed=0, hidden=HID):
-        assert hidden == HID
+        if hidden != HID:
+            raise ValueError(f"the mesh-convolution kernels are built for {HID} hidden channels, not {hidden}")
         g = torch.Generator().manual_seed(seed)
         d = device()
         def init(cout, cin, fan_in, gain):

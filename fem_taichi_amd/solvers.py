@@ -1,7 +1,7 @@
 """GPU static solvers on the device-resident BSR(3x3) stiffness.
 
-These replace the Pardiso factor/solve of SOLVE_static.py:31-38 (which is
-kept only as the CPU oracle).  Every form is measured by the FP64 relative
+These replace the Pardiso factor/solve of SOLVE_static.py:31-38 (the tests
+check them against a direct sparse solve on the CPU).  Every form is measured by the FP64 relative
 residual ||F - K U|| / ||F|| on the penalised K, as the reference prints it
 (SOLVE_static.py:36-38):
 
