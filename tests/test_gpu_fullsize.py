@@ -7,7 +7,9 @@ need no oracle run at this size.
 - the unpenalised K annihilates the six rigid-body modes (translations and
   infinitesimal rotations: their discrete strain is exactly zero), is
   symmetric, and the penalised K is positive on random vectors;
-- the consistent mass of the unjittered 60x60x55 box sums to rho * V;
+- the FP32 K is within 1e-6 of the FP64 K;
+- the consistent mass of the unjittered 60x60x55 box sums to rho * V, and
+  its centrifugal load vector to rho |omega|^2 V (xc, yc, 0);
 - an affine displacement u = G x (reproduced exactly by trilinear elements)
   gives each node the stress E_bar * (lhat tr(eps) I + 2 mhat eps), eps =
   sym(G), with E_bar the mean modulus of the node's elements.
@@ -182,3 +184,26 @@ def test_blade_modes(blade):
     res = (lam_t * (Y - Xd / lam_t).norm(dim=0) / Xd.norm(dim=0)).cpu().numpy()
     print("shift-invert residuals", res, flush=True)
     assert np.all(res < 1e-6), res
+
+
+def test_f32_k_within_stated_bound(slab):
+    """The FP32 K of the full slab against its FP64 K: 1e-6 relative."""
+    import torch
+    g, phys, model = slab
+    k64 = model.assemble_K("f64", "bsr", penalty=False)
+    k32 = model.assemble_K("f32", "bsr", penalty=False).double()
+    assert float((k32 - k64).norm() / k64.norm()) < 1e-6
+
+
+def test_box_centrifugal_load_total(gpu):
+    """Sum of the load vector of the unjittered 60x60x55 box spinning at
+    omega = (0, 0, 100) rad/s, no nodal loads: rho * |omega|^2 * V * (xc, yc, 0),
+    with (xc, yc) the box centre (the partition of unity sums the element
+    integrals exactly)."""
+    from fem_taichi_amd import Model, Physics
+    g = mesh.structured_block(60, 60, 55, 1.0, 0.0, 1, load_face=None, omegas=(0.0, 0.0, 100.0))
+    model = Model(g.name, Physics.from_grid(g), g.Materials, Omegas=g.Omegas)
+    F = model.F_device().view(-1, 3).sum(0).cpu().numpy()
+    rv = g.Materials["DE"] * 100.0 ** 2 * 60 * 60 * 55
+    expect = np.array([rv * 30.0, rv * 30.0, 0.0])
+    assert np.linalg.norm(F - expect) < 1e-12 * np.linalg.norm(expect)
