@@ -13,7 +13,7 @@ when num_block does not divide 2*num_eigen.  Differences, each stated:
 the inner solves K^-1 (M u) are iterative GPU solves to ``inner_tol``
 instead of a Pardiso factorisation (FP64 flexible CG with the FP32 multigrid
 V-cycle on structured grids, multi-RHS block-Jacobi PCG otherwise); the start block is drawn from
-``np.random.default_rng(seed)`` (the reference's is unseeded); the dense
+``np.random.default_rng(seed)``, or passed in as ``u0`` (the reference's is unseeded); the dense
 b x b / m x m work (Cholesky, inverse, eigh of T) and the tall-by-small
 products Q C run as FP64 library calls on the device; the inner products
 X^T Y over the 3N rows (the M-Gram blocks and the reorthogonalisation
@@ -209,7 +209,11 @@ def _chol_upper(A):
 def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     """SOLVE_eigen.py:45-89.  ``ops`` provides ``n``, ``device``,
     ``m_apply(X)`` and ``solve(B, tol)``.  Returns (eigenvalues ascending
-    [:num_eigen], vectors (n, num_eigen), T) as torch tensors."""
+    [:num_eigen], vectors (n, num_eigen), T) as torch tensors.
+
+    The start block is ``u0`` (n, b) when given, else uniform [0, 1) draws
+    of np.random.default_rng(seed) (the reference draws it unseeded with
+    np.random.uniform, SOLVE_eigen.py:51)."""
     d = ops.device
     m = num_eigen * 2
     n = ops.n
@@ -261,7 +265,7 @@ class Eigen_solver:
         self.ops = None
 
     def Solve_cpu(self, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, verbose=False, inner="auto",
-                  prestress=None, omega=None):
+                  prestress=None, omega=None, u0=None):
         """``inner``: "mg" (multigrid flexible CG, structured grids), "pcg"
         (multi-RHS block-Jacobi PCG) or "auto" (mg when the Physics has
         node_dims).
@@ -306,7 +310,7 @@ class Eigen_solver:
             return Y
 
         self.ops = MultigridBlockOps(Mo, m_apply, k_vals=kv) if inner == "mg" else BlockOps(dev_mesh, kv, m_apply)
-        lam, X, T = lanczos(self.ops, num_eigen, num_block, seed=seed, inner_tol=inner_tol)
+        lam, X, T = lanczos(self.ops, num_eigen, num_block, seed=seed, inner_tol=inner_tol, u0=u0)
         self.eigenvalues = lam
         freq = torch.sqrt(lam) / 2 / np.pi
         if verbose:
@@ -325,7 +329,7 @@ class Block_lancozs:
         self.K, self.M, self.K_inv = K, M, K_inv
         self.num_eigen, self.num_block = num_eigen, num_block
 
-    def Solve_cpu(self, seed=0, inner_tol=1e-12):
+    def Solve_cpu(self, seed=0, inner_tol=1e-12, u0=None):
         Kb = self.K.tobsr(blocksize=(3, 3))
         Kb.sort_indices()
         Mb = self.M.tobsr(blocksize=(3, 3))
@@ -345,5 +349,5 @@ class Block_lancozs:
             return Y
 
         ops = BlockOps(kmesh, kv, m_apply)
-        lam, X, _ = lanczos(ops, self.num_eigen, self.num_block, seed=seed, inner_tol=inner_tol)
+        lam, X, _ = lanczos(ops, self.num_eigen, self.num_block, seed=seed, inner_tol=inner_tol, u0=u0)
         return lam.cpu().numpy(), X.cpu().numpy()

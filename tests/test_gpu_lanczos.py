@@ -36,7 +36,7 @@ def test_eigen_solver_matches_oracle(cantilever):
     lu = sla.splu(K)
     lam_ref, X_ref, _ = O.lanczos(K, M, lambda B: lu.solve(B), num_eigen, b, u0=u0)
     solver = Eigen_solver(model, 0)
-    freq, X = solver.Solve_cpu(num_eigen=num_eigen, num_block=b, seed=0)
+    freq, X = solver.Solve_cpu(num_eigen=num_eigen, num_block=b, u0=u0)     # the oracle's start block
     lam = solver.eigenvalues.cpu().numpy()
     # compare the Ritz pairs the oracle shows converged, by the shift-invert
     # residual lam * ||K^-1 M x - x / lam|| / ||x|| <= 1e-4 (the plain K
