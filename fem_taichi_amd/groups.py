@@ -97,6 +97,19 @@ def build_row_groups(N, E, ne_ptr, ne_list, gid, max_elems, co_node=None):
                 max_group_incs=int(incs.max()) if G else 0)
 
 
+# Order of the tasks inside a group: class (20 - log2 T for blocks of T
+# lanes, decreasing T; 21 for the dual tasks after them), then the block (or
+# dual pair) index, then the lane within the block.  Packed into one int64 as
+# class * 2^52 + index * 64 + lane: the index may be any block number of the
+# mesh (< 2^46), so the class always dominates.
+_KEY_CLASS = 1 << 52
+
+
+def task_sort_key(cls, idx, lane):
+    """Sort key of a task (numpy or torch int64 arrays, or ints)."""
+    return cls * _KEY_CLASS + idx * 64 + lane
+
+
 def _next_pow2(a):
     a = np.maximum(a, 1)
     return (1 << np.ceil(np.log2(a)).astype(np.int64)).astype(np.int64)
@@ -184,8 +197,7 @@ def build_tasks(N, elem, sm_node, re_node, ne_ptr, ne_list, groups):
         lt = np.log2(T[tb]).astype(np.int64)
         w2 = c1 | (c2 << 14) | (lt << 28)
         # decreasing T inside a group keeps every block's lanes T-aligned
-        sortk = (20 - lt) * (1 << 24) + tb
-        recs.append((sg[tb], sortk * 64 + q, sout[tb], np.full(len(tb), -1), w2))
+        recs.append((sg[tb], task_sort_key(20 - lt, tb, q), sout[tb], np.full(len(tb), -1), w2))
     ss = np.nonzero(~multi)[0]
     if len(ss):
         # pair singles of the same group into dual tasks
@@ -203,8 +215,7 @@ def build_tasks(N, elem, sm_node, re_node, ne_ptr, ne_list, groups):
         oB = np.where(b_ok, sout[ss[np.minimum(b_idx, len(ss) - 1)]], -1)
         w2 = cA | (cB << 14) | (1 << 31)
         # after every multi-task block (T = 1 needs no alignment)
-        sortk = np.full(len(a_idx), 21 << 30, dtype=np.int64) + a_idx
-        recs.append((gs[a_idx], sortk, sout[ss[a_idx]], oB, w2))
+        recs.append((gs[a_idx], task_sort_key(21, a_idx.astype(np.int64), 0), sout[ss[a_idx]], oB, w2))
     gg = np.concatenate([x[0] for x in recs])
     sk = np.concatenate([x[1] for x in recs])
     oa = np.concatenate([x[2] for x in recs])
@@ -303,8 +314,7 @@ def build_tasks_torch(N, elem, sm_node, re_node, ne_ptr, ne_list, groups, device
         c2 = torch.where(has2, cc[torch.clamp(i2, max=last)], torch.full_like(i2, NONE))
         lt = torch.round(torch.log2(T[tb].double())).to(i64)
         w2 = c1 | (c2 << 14) | (lt << 28)
-        sortk = (20 - lt) * (1 << 24) + tb
-        recs.append((sg[tb], sortk * 64 + q, sout[tb], torch.full_like(tb, -1), w2))
+        recs.append((sg[tb], task_sort_key(20 - lt, tb, q), sout[tb], torch.full_like(tb, -1), w2))
     ss = torch.nonzero(~multi).squeeze(1)
     if ss.numel():
         gs = sg[ss]
@@ -320,8 +330,7 @@ def build_tasks_torch(N, elem, sm_node, re_node, ne_ptr, ne_list, groups, device
         cB = torch.where(b_ok, cc[start[ss[torch.clamp(b_idx, max=lastb)]]], torch.full_like(a_idx, NONE))
         oB = torch.where(b_ok, sout[ss[torch.clamp(b_idx, max=lastb)]], torch.full_like(a_idx, -1))
         w2 = cA | (cB << 14) | (1 << 31)
-        sortk = torch.full_like(a_idx, 21 << 30) + a_idx
-        recs.append((gs[a_idx], sortk, sout[ss[a_idx]], oB, w2))
+        recs.append((gs[a_idx], task_sort_key(21, a_idx, 0), sout[ss[a_idx]], oB, w2))
     gg = torch.cat([x[0] for x in recs])
     sk = torch.cat([x[1] for x in recs])
     oa = torch.cat([x[2] for x in recs])

@@ -492,14 +492,17 @@ def _colnorms(A):
 
 
 def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
-                       drop=1e-2):
+                       drop=1e-2, check_after=50):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
     FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) on the
     FP32 copy of the residual, one FP64 SpMM (fem_bsr_spmm) and the fused
     vector steps (FcgSteps) per iteration.  Each column stops on its own
     (tolerance, or the attainable accuracy as in solve_fcg_mg) and is frozen
-    from then on.  Returns (X, info) with per-column ``converged``,
-    ``stagnated`` and ``rel_residual``."""
+    from then on.  A column whose recursive residual has not dropped by
+    ``drop`` within ``check_after`` iterations of its last true-residual
+    check gets one anyway, so a residual that levels off above ``tol`` still
+    meets the stagnation test.  Returns (X, info) with per-column
+    ``converged``, ``stagnated`` and ``rel_residual``."""
     from .learned import free_dof_mask
     st = stream_ptr()
     n, nb = B.shape
@@ -532,6 +535,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     anchor = _colnorms(R)
     best = np.full(nb, np.inf)
     stalls = np.zeros(nb, dtype=int)
+    last_check = np.zeros(nb, dtype=int)
     act_d = torch.as_tensor(active.astype(np.int32), device=B.device)
     steps = FcgSteps(n, nb, B.device)
     P = torch.empty_like(B)
@@ -549,7 +553,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
         rf = np.sqrt(rr.cpu().numpy())
         if not np.all(np.isfinite(rf[active])):
             break
-        need = active & ((rf < drop * anchor) | (rf < tol * bn))
+        need = active & ((rf < drop * anchor) | (rf < tol * bn) | (it - last_check >= check_after))
         if need.any():
             # true residuals of the columns that asked for one (one or two
             # of them by single-vector products)
@@ -569,6 +573,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
                 R32[:, ci] = Rt.float()
             for c in cols:
                 anchor[c] = tn[c]
+                last_check[c] = it
                 if tn[c] < tol * bn[c]:
                     converged[c] = True
                     active[c] = False

@@ -211,7 +211,8 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     ``m_apply(X)`` and ``solve(B, tol)``.  Returns (eigenvalues ascending
     [:num_eigen], vectors (n, num_eigen), T) as torch tensors.
 
-    The start block is ``u0`` (n, b) when given, else uniform [0, 1) draws
+    The start block is ``u0`` (n, b; an array or a device tensor) when
+    given, else uniform [0, 1) draws
     of np.random.default_rng(seed) (the reference draws it unseeded with
     np.random.uniform, SOLVE_eigen.py:51)."""
     d = ops.device
@@ -222,7 +223,10 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     T = torch.zeros(m, m, dtype=torch.float64, device=d)
     if u0 is None:
         u0 = np.random.default_rng(seed).uniform(size=(n, b))
-    u = torch.as_tensor(np.asarray(u0, dtype=np.float64)).to(d)
+    if isinstance(u0, torch.Tensor):
+        u = u0.to(device=d, dtype=torch.float64).contiguous()
+    else:
+        u = torch.as_tensor(np.asarray(u0, dtype=np.float64)).to(d)
     v = _chol_upper(gram(u, ops.m_apply(u)))
     u = u @ torch.linalg.inv(v)
     Q[:, :b] = u

@@ -207,3 +207,35 @@ def test_box_centrifugal_load_total(gpu):
     rv = g.Materials["DE"] * 100.0 ** 2 * 60 * 60 * 55
     expect = np.array([rv * 30.0, rv * 30.0, 0.0])
     assert np.linalg.norm(F - expect) < 1e-12 * np.linalg.norm(expect)
+
+
+def test_blade_task_lanes_aligned(blade):
+    """Every block of the blade's device-built task lists (27.5M node-pair
+    blocks, beyond 2^24) occupies exactly T adjacent, T-aligned lanes of its
+    group, T non-increasing inside a group and the dual tasks last; the
+    lane butterfly sums a block correctly only then."""
+    import torch
+    g, phys, model = blade
+    t = phys.dev.t
+    tp = t["grp_task_ptr"].long()
+    w = t["tasks"].view(-1, 2).long() & 0xFFFFFFFF
+    nt = w.shape[0]
+    assert int(tp[-1]) == nt
+    dual = ((w[:, 1] >> 31) & 1).bool()
+    lt = (w[:, 1] >> 28) & 7
+    T = torch.where(dual, torch.ones_like(lt), 1 << lt)
+    outa = w[:, 0] & 0xFFFF
+    G = tp.numel() - 1
+    grp = torch.repeat_interleave(torch.arange(G, device=w.device), tp.diff())
+    pos = torch.arange(nt, device=w.device) - tp[grp]
+    start = torch.ones(nt, dtype=torch.bool, device=w.device)
+    start[1:] = (grp[1:] != grp[:-1]) | (outa[1:] != outa[:-1]) | dual[1:] | dual[:-1]
+    run = torch.cumsum(start.long(), 0) - 1
+    length = torch.bincount(run)
+    first = torch.nonzero(start).squeeze(1)
+    assert torch.equal(length, T[first])                       # T tasks per block
+    assert bool((pos[first] % T[first] == 0).all())           # T-aligned
+    assert bool((T == T[first][run]).all())
+    cls = torch.where(dual, torch.full_like(lt, 21), 20 - lt)  # 20 - log2 T, then the dual tasks
+    same = grp[1:] == grp[:-1]
+    assert bool((cls[1:][same] >= cls[:-1][same]).all())

@@ -276,3 +276,21 @@ def test_tasks_torch_equal_numpy_crowded_and_mesh2():
         a = Gr.build_tasks(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
         b = Gr.build_tasks_torch(N, en, pp["SmNode"], pp["ReNode"], ne_ptr, ne_list, groups)
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_task_sort_key_orders_by_class_first_at_any_block_index():
+    """Task order inside a group is (class, block index, lane); block indices
+    run over the whole mesh, beyond 2^24 blocks on million-node meshes, and
+    must never spill into the class (a packed key that let them did
+    misalign the T-lane blocks of a group straddling 2^24 blocks)."""
+    rng = np.random.default_rng(0)
+    n = 200000
+    cls = rng.integers(15, 22, n)
+    idx = rng.integers(0, 1 << 40, n)
+    idx[:1000] = (1 << 24) + rng.integers(-8, 8, 1000)     # around the old spill point
+    lane = rng.integers(0, 32, n)
+    key = Gr.task_sort_key(cls, idx, lane)
+    assert np.array_equal(np.argsort(key, kind="stable"), np.lexsort((lane, idx, cls)))
+    import torch
+    kt = Gr.task_sort_key(torch.as_tensor(cls), torch.as_tensor(idx), torch.as_tensor(lane))
+    assert np.array_equal(kt.numpy(), key)
