@@ -12,8 +12,9 @@ the block tridiagonal T, lambda = 1/theta, ascending order, zero-padded T
 when num_block does not divide 2*num_eigen.  Differences, each stated:
 the inner solves K^-1 (M u) are iterative GPU solves to ``inner_tol``
 instead of a Pardiso factorisation (FP64 flexible CG with the FP32 multigrid
-V-cycle on structured grids, multi-RHS block-Jacobi PCG otherwise); the start block is drawn from
-``np.random.default_rng(seed)``, or passed in as ``u0`` (the reference's is unseeded); the dense
+V-cycle on structured grids, multi-RHS block-Jacobi PCG otherwise); the start block is drawn on
+the device from a generator seeded with ``seed``, or passed in as ``u0`` (the reference's is
+unseeded); the dense
 b x b / m x m work (Cholesky, inverse, eigh of T) and the tall-by-small
 products Q C run as FP64 library calls on the device; the inner products
 X^T Y over the 3N rows (the M-Gram blocks and the reorthogonalisation
@@ -212,9 +213,10 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     [:num_eigen], vectors (n, num_eigen), T) as torch tensors.
 
     The start block is ``u0`` (n, b; an array or a device tensor) when
-    given, else uniform [0, 1) draws
-    of np.random.default_rng(seed) (the reference draws it unseeded with
-    np.random.uniform, SOLVE_eigen.py:51)."""
+    given, else uniform [0, 1) draws from a device generator seeded with
+    ``seed`` (the reference draws it unseeded with np.random.uniform,
+    SOLVE_eigen.py:51); on the device because a host draw of a 7M x 16 block
+    alone takes seconds."""
     d = ops.device
     m = num_eigen * 2
     n = ops.n
@@ -222,8 +224,10 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     Q = torch.zeros(n, m, dtype=torch.float64, device=d)
     T = torch.zeros(m, m, dtype=torch.float64, device=d)
     if u0 is None:
-        u0 = np.random.default_rng(seed).uniform(size=(n, b))
-    if isinstance(u0, torch.Tensor):
+        gen = torch.Generator(device=d)
+        gen.manual_seed(int(seed))
+        u = torch.rand(n, b, dtype=torch.float64, device=d, generator=gen)
+    elif isinstance(u0, torch.Tensor):
         u = u0.to(device=d, dtype=torch.float64).contiguous()
     else:
         u = torch.as_tensor(np.asarray(u0, dtype=np.float64)).to(d)
