@@ -17,7 +17,7 @@ The JSON line also carries:
                    one warm-up run, on this host's cores;
   f32           -- the same assembly into an FP32 BSR copy;
   static_solve  -- seconds to an FP64-verified 1e-8 relative residual on the
-                   rotating twisted blade 400x50x50 (1,040,451 nodes), per
+                   rotating twisted blade 400x50x50 (1,043,001 nodes), per
                    solver form: FP64 block-Jacobi PCG, FP64 flexible CG with
                    the FP32 multigrid V-cycle, and the learned form (the
                    V-cycle plus the mesh-convolution correction), with the

@@ -18,6 +18,8 @@ import numpy as np
 # 60x60x55 slab and the 400x50x50 blade (measured; the values are identical).
 DEFAULT_BRICK = (2, 4, 2)
 DEFAULT_MORTON_ROWS = 8
+# max_group_slots when the row block counts are not given (int32 maximum)
+MAX_SLOTS_UNKNOWN = 2 ** 31 - 1
 
 
 def _morton_order(xyz: np.ndarray, bits: int = 10) -> np.ndarray:
@@ -87,13 +89,16 @@ def build_row_groups(N, E, ne_ptr, ne_list, gid, max_elems, co_node=None):
     grp_elem_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
     grp_elems = (uniq % max(E, 1)).astype(np.int32)
     ne_local = (np.searchsorted(uniq, keys) - grp_elem_ptr[gid[inc_node]]).astype(np.int32)
-    slots = np.bincount(gid, weights=co_node, minlength=G) if (co_node is not None and G) else np.zeros(1)
+    # without CoNode the slot count is unknown: report the int32 maximum, which
+    # keeps the assembly's LDS-staged copy-out off (it needs every group's
+    # blocks to fit its gradient area)
+    slots = np.bincount(gid, weights=co_node, minlength=G) if (co_node is not None and G) else None
     incs = np.bincount(gid, weights=np.diff(ne_ptr), minlength=G) if G else np.zeros(1)
     return dict(grp_row_ptr=grp_row_ptr, grp_rows=order.astype(np.int32), grp_elem_ptr=grp_elem_ptr,
                 grp_elems=grp_elems, ne_local=ne_local, num_groups=G,
                 max_group_elems=int(counts.max()) if G else 0,
                 max_group_rows=int(np.diff(grp_row_ptr).max()) if G else 0,
-                max_group_slots=int(slots.max()) if G else 0,
+                max_group_slots=(int(slots.max()) if slots is not None else MAX_SLOTS_UNKNOWN) if G else 0,
                 max_group_incs=int(incs.max()) if G else 0)
 
 

@@ -66,10 +66,10 @@ class Model:
         self.name = name
         self.DE, self.EX, self.PO = Materials['DE'], Materials['EX'], Materials['PO']
         self.OM = np.array(Omegas, dtype=np.float64)
+        from .physics import Physics as _Physics
+        Physics = _Physics.adopt(Physics)      # also takes the reference's Read object
         self.Physics = Physics
         self.Largedisp = Largedisp
-        if getattr(Physics, "dev", None) is None:
-            Physics.pre_process()
         self.dev = Physics.dev
         self.Num_Node = Physics.Num_Node
         self.Num_Elem = Physics.Num_Elem
@@ -102,7 +102,7 @@ class Model:
         np.add.at(load, nind[fid], forces[1:4].T)
         self.nodal_load = torch.as_tensor(load.reshape(-1)).to(d)
         self._cache = {}
-        self._csr_k_host = None
+        self._csr_k_dev = None
 
     # ---------------------------------------------------------------- device forms
     def _young_ptr(self):
@@ -230,15 +230,19 @@ class Model:
         return sp.csr_matrix((_to_host(k), col, row), shape=(n, n), dtype=np.float64)
 
     def _csr_index_K(self):
-        """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173), host copies,
-        built once per Model (they depend on the pattern only) and shared by the matrices returned."""
-        if self._csr_k_host is None:
+        """RowK / ColK of the reference's CSR layout (FEM_model_csr.py:163, 168-173) as fresh host
+        arrays.  They depend on the pattern only, so they are built on the device once per Model and
+        kept there; every returned matrix gets its own host copy (as the reference's to_numpy() does),
+        so a caller that edits one matrix's structure in place (eliminate_zeros, sort) leaves every
+        other matrix alone."""
+        if self._csr_k_dev is None:
             d = device()
             row = torch.empty(3 * self.Num_Node + 1, dtype=torch.int32, device=d)
             col = torch.empty(max(self.nnz_K, 1), dtype=torch.int32, device=d)
             _lib.call("fem_csr_index_K", self.dev.ref, row.data_ptr(), col.data_ptr(), stream_ptr())
-            self._csr_k_host = (_to_host(row), _to_host(col[:self.nnz_K]))
-        return self._csr_k_host
+            self._csr_k_dev = (row, col[:self.nnz_K])
+        row, col = self._csr_k_dev
+        return _to_host(row), _to_host(col)
 
     def Get_MatrixKL(self):
         """FEM_model_csr.py:118-127: penalised stiffness as scipy CSR (3N x 3N)."""

@@ -492,7 +492,7 @@ def _colnorms(A):
 
 
 def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
-                       drop=1e-2, check_after=50):
+                       drop=1e-2, check_after=50, floor=1e-7):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
     FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) on the
     FP32 copy of the residual, one FP64 SpMM (fem_bsr_spmm) and the fused
@@ -501,7 +501,12 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     from then on.  A column whose recursive residual has not dropped by
     ``drop`` within ``check_after`` iterations of its last true-residual
     check gets one anyway, so a residual that levels off above ``tol`` still
-    meets the stagnation test.  Returns (X, info) with per-column
+    meets the stagnation test.  A column counts as at its attainable FP64
+    accuracy (``stagnated``) only when ``patience`` checks in a row fail to
+    halve its best true residual AND that residual is below ``floor`` times
+    its right-hand side; a column that stalls above ``floor`` stays active,
+    so it runs to ``maxiter`` and is neither converged nor stagnated (the
+    caller's fallback finishes it).  Returns (X, info) with per-column
     ``converged``, ``stagnated`` and ``rel_residual``."""
     from .learned import free_dof_mask
     st = stream_ptr()
@@ -581,7 +586,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
                     best[c], stalls[c] = tn[c], 0
                 else:
                     stalls[c] += 1
-                    if stalls[c] >= patience:
+                    if stalls[c] >= patience and tn[c] < floor * bn[c]:
                         stagnated[c] = True
                         active[c] = False
             act_d.copy_(torch.as_tensor(active.astype(np.int32)))

@@ -108,6 +108,34 @@ class Physics:
             self.pre_process()
 
     @classmethod
+    def adopt(cls, obj):
+        """A Physics for ``obj``: ``obj`` itself when it is one, else a Physics
+        built from the attributes the reference's ``Read`` carries
+        (READ_physics.py:44-61, 96-97, 114, 134: Node, Elem, Bounds, Forces,
+        name), with the pattern on the GPU.  The reference's SciPy
+        ``pre_process`` sets no device state, so a ``Read`` handed to ``Model``
+        or ``Post`` is wrapped here; the wrapper is kept on the object (as
+        ``_fem355_physics``) so Model and Post of one Read share one pattern."""
+        if isinstance(obj, cls):
+            if obj.dev is None:
+                obj.pre_process()
+            return obj
+        cached = getattr(obj, "_fem355_physics", None)
+        if isinstance(cached, cls) and cached._src[0] is obj.Node and cached._src[1] is obj.Elem:
+            return cached
+        for attr in ("Node", "Elem"):
+            if not hasattr(obj, attr):
+                raise TypeError(f"{type(obj).__name__} has no {attr}: not a Read-shaped mesh")
+        phys = cls(obj.Node, obj.Elem, getattr(obj, "Bounds", None), getattr(obj, "Forces", None),
+                   name=getattr(obj, "name", "mesh"), node_dims=getattr(obj, "node_dims", None))
+        phys._src = (obj.Node, obj.Elem)      # a wrapper is reused only while these are unchanged
+        try:
+            obj._fem355_physics = phys
+        except AttributeError:
+            pass
+        return phys
+
+    @classmethod
     def from_grid(cls, grid, **kw):
         """From a ``mesh.StructuredGrid`` (keeps its brick-able node layout)."""
         return cls(grid.Node, grid.Elem, grid.Bounds, grid.Forces, name=grid.name,

@@ -27,9 +27,9 @@ class Post:
         self.name = name
         self.path = None           # Results/<name>/, created by the writers
         self.DE, self.EX, self.PO = Materials['DE'], Materials['EX'], Materials['PO']
+        from .physics import Physics as _Physics
+        Physics = _Physics.adopt(Physics)      # also takes the reference's Read object
         self.Physics = Physics
-        if getattr(Physics, "dev", None) is None:
-            Physics.pre_process()
         self.Num_Node = Physics.Node.shape[-1]
         self.Num_Elem = Physics.Elem.shape[-1]
         if isinstance(Solutions, torch.Tensor):

@@ -151,6 +151,8 @@ class MultigridBlockOps:
         self.n = 3 * model.Num_Node
         self.device = device()
         self.iterations = []
+        self.inner_rel_max = 0.0      # worst relative residual ||B - K X|| / ||B|| an inner solve returned
+        self.fallback_columns = 0
 
     def solve(self, B, tol, maxiter=1000):
         from . import multigrid, solvers
@@ -160,11 +162,17 @@ class MultigridBlockOps:
                                                maxiter=maxiter, free=self.free, dinv64=self.dinv64,
                                                patience=self.patience)
         total = info["iterations"] * B.shape[1]
+        rel = np.array(info["rel_residual"], dtype=np.float64)
         for j in np.nonzero(~(info["converged"] | info["stagnated"]))[0]:
-            x, it2, _ = solvers.pcg(self.K64, B[:, j].contiguous(), self.dinv64, tol, 200000,
-                                    x0=X[:, j].contiguous().clone())
+            bj = B[:, j].contiguous()
+            x, it2, _ = solvers.pcg(self.K64, bj, self.dinv64, tol, 200000, x0=X[:, j].contiguous().clone())
             X[:, j] = x
             total += it2
+            self.fallback_columns += 1
+            bnj = float(torch.linalg.vector_norm(bj))
+            rel[j] = float(torch.linalg.vector_norm(bj - self.K64.matvec(x))) / bnj if bnj > 0 else 0.0
+        if rel.size:
+            self.inner_rel_max = max(self.inner_rel_max, float(np.max(rel)))
         self.iterations.append(total)
         return X
 

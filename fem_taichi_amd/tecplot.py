@@ -164,7 +164,12 @@ def field_table(node_xyz, fields, field_names):
 def write_dat(path, title, node_xyz, fields, field_names, elem1):
     """The reference's ASCII form (POST_process.py:101-135): a TITLE line, the
     VARIABLES line as it writes it, a FEPOINT BRICK zone line, the node rows
-    (%10.5e, space separated) and the 1-based element rows."""
+    (%10.5e, space separated) and the 1-based element rows.
+
+    POST_process.py:125 writes ``"%s " % ',"' + name + str(step + 1) + '"'``:
+    ``%`` binds tighter than ``+``, so each field comes out as ``," u1"`` (the
+    space after the comma-quote, none after the name), and that is what is
+    written here."""
     fields = np.asarray(fields, dtype=np.float64)
     N, F, S = fields.shape
     data = np.concatenate([np.asarray(node_xyz, dtype=np.float64),
@@ -174,7 +179,7 @@ def write_dat(path, title, node_xyz, fields, field_names, elem1):
         f.write('VARIABLES = "X [ m ]","Y [ m ]","Z [m]"')
         for step in range(S):
             for name in field_names:
-                f.write('%s ' % (',"' + str(name) + str(step + 1) + '"'))
+                f.write(('%s ' % ',"') + str(name) + str(step + 1) + '"')
         f.write("\n")
         f.write("ZONE  I=%d, J=%d, F=FEPOINT, ET=BRICK\n" % (N, elem1.shape[0]))
         np.savetxt(f, data, fmt="%10.5e", delimiter=" ")

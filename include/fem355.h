@@ -82,7 +82,11 @@ typedef struct fem_mesh {
   int64_t num_groups;
   int32_t max_group_elems;     /* max over groups of the element count */
   int32_t max_group_rows;      /* max over groups of the row count     */
-  int32_t max_group_slots;     /* max over groups of sum CoNode of its rows */
+  int32_t max_group_slots;     /* max over groups of sum CoNode of its rows; never
+                                  less than the true value (fem_hex8_assemble sizes its
+                                  LDS-staged copy-out by it, so an understated value is
+                                  undefined behaviour); INT32_MAX when unknown, which
+                                  turns the staging off */
   int32_t max_group_tasks;     /* max over groups of the task count */
   const int32_t* grp_row_ptr;  /* G + 1 */
   const int32_t* grp_rows;     /* N, rows in group order */

@@ -22,10 +22,11 @@ assemble_M_csr         FEM_model_csr.py:178-192                    brute-force d
 modify_K               FEM_model_csr.py:233-242 / bsr:248-259      exact 1e20 set (not add)
 vector_F               FEM_model_csr.py:141-146, 205-229           linearity, closed form
 get_stress             POST_process.py:72-99, 172-223, 277-321     uniform-strain patch test (D,B);
-                                                                   eigh; the Q3 weight order and
-                                                                   the Q4 von Mises term: parity
-                                                                   unpinned (restated from the
-                                                                   source text only)
+                                                                   eigh; Q3: the extrapolation map
+                                                                   against a hand-counted table and
+                                                                   a unit cube with hand-derived
+                                                                   Gauss-point stresses; Q4: hand
+                                                                   principal triples
 lanczos                SOLVE_eigen.py:45-89                        dense scipy eigh on small pencils,
                                                                    the SOLVE_eigen.py:101-117 pencil
 element_KN             FEM_model_bsr.py:476-598 (UnitK, BN, S)     U = 0 gives element_K; rigid
@@ -45,10 +46,11 @@ Q4 the von Mises term ``(p2-p1)^2``; Q6 zero-padded T in block Lanczos; QN
 the geometric stiffness of UnitK uses the element stress accumulated over the
 Gauss points visited so far (S is an element field summed in the Gauss loop).
 
-Parity of the extrapolation map of Q3 and of the von Mises quirk Q4 cannot be
-pinned by a closed form (a patch test is blind to them: constant fields pass
-through any partition of unity); they are pinned only by this restatement of
-the source text.
+A patch test is blind to the extrapolation map of Q3 (constant fields pass
+through any partition of unity), so Q3 is pinned by values counted by hand
+from the two sign tables of POST_process.py:215-222 and :310-317, and on a
+unit cube whose Gauss-point stresses are derived by hand; Q4 by principal
+triples whose quirk value is computed by hand (tests/test_oracle.py).
 """
 from __future__ import annotations
 

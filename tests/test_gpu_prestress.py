@@ -82,6 +82,9 @@ def test_prestressed_modal_of_rotating_blade(gpu):
     s = Eigen_solver(model, 0)
     s.Solve_cpu(num_eigen=12, num_block=4, seed=1, prestress="static")
     assert s.operator == "KS"
+    # no inner solve handed back a column stalled above the FP64 floor
+    print("worst inner residual", s.ops.inner_rel_max, "fallback columns", s.ops.fallback_columns, flush=True)
+    assert s.ops.inner_rel_max < 1e-7
     lam = s.eigenvalues.cpu().numpy()
     print("prestressed", lam[:4], "eigsh", ref, "unstressed eigsh", ref0, flush=True)
     assert np.allclose(lam[:3], ref[:3], rtol=EIG_TOL, atol=0)
@@ -109,6 +112,8 @@ def test_modal_after_newton_uses_tangent(gpu):
     s = Eigen_solver(model, 0)
     s.Solve_cpu(num_eigen=12, num_block=4, seed=2)
     assert s.operator == "KT"
+    print("worst inner residual", s.ops.inner_rel_max, "fallback columns", s.ops.fallback_columns, flush=True)
+    assert s.ops.inner_rel_max < 1e-7
     lam = s.eigenvalues.cpu().numpy()
     print("KT modes", lam[:4], "eigsh", ref, flush=True)
     assert np.allclose(lam[:3], ref[:3], rtol=EIG_TOL, atol=0)

@@ -258,14 +258,128 @@ def test_stress_uniform_strain_patch():
     assert np.allclose(S[:, 9], vm, rtol=1e-10)
 
 
+# Q3 by hand.  POST_process.py:177-181 gives node j the weight
+# Ns[k] = shapefunc_GS(GMat[j])[k] on SVec[k] (the stress at Gauss point
+# RMat[k]).  shapefunc_GS (:215-222) writes factor d of Ns[k] as
+# (1 +- rtr[d]/GS) with the sign of the shape-node table, and GMat[j]
+# (:310-317) lists the corners (-1,-1,-1), (-1,-1,+1), (-1,+1,-1), ... So
+# Ns[k] = (1 + 1/GS)^m (1 - 1/GS)^(3-m) / 8, m = the number of axes on which
+# GMat[j] and the k-th sign row of shapefunc_GS agree.  The m table below was
+# counted by hand from those two source tables:
+#   shapefunc_GS rows   k0 (+,+,-) k1 (-,+,-) k2 (-,-,-) k3 (+,-,-)
+#                       k4 (+,+,+) k5 (-,+,+) k6 (-,-,+) k7 (+,-,+)
+_Q3_AGREE = np.array([
+    [1, 2, 3, 2, 0, 1, 2, 1],   # GMat[0] (-,-,-)
+    [0, 1, 2, 1, 1, 2, 3, 2],   # GMat[1] (-,-,+)
+    [2, 3, 2, 1, 1, 2, 1, 0],   # GMat[2] (-,+,-)
+    [1, 2, 1, 0, 2, 3, 2, 1],   # GMat[3] (-,+,+)
+    [2, 1, 2, 3, 1, 0, 1, 2],   # GMat[4] (+,-,-)
+    [1, 0, 1, 2, 2, 1, 2, 3],   # GMat[5] (+,-,+)
+    [3, 2, 1, 2, 2, 1, 0, 1],   # GMat[6] (+,+,-)
+    [2, 1, 0, 1, 3, 2, 1, 2],   # GMat[7] (+,+,+)
+])
+
+
+def _q3_by_hand():
+    a, b = 1.0 + 1.0 / 0.57735026918962, 1.0 - 1.0 / 0.57735026918962
+    return a ** _Q3_AGREE * b ** (3 - _Q3_AGREE) / 8.0
+
+
 def test_extrapolation_weights_quirk():
-    """Q3: W[j] = shapefunc_GS(GMat[j]); rows sum to 1, but W is not the
-    textbook corner extrapolation (whose Gauss point k sits at RMat[k])."""
+    """Q3: W[j] = shapefunc_GS(GMat[j]) equals the hand-counted table entry for
+    entry, e.g. W[0, 2] = (1 + 1/GS)^3 / 8 (GMat[0] and row k2 agree on all
+    three axes).  Swapping GMat for the shape-node corners, or evaluating
+    shapefunc_GS in RMat order, changes rows and fails here."""
     W = O.extrapolation_weights()
-    assert np.allclose(W.sum(axis=1), 1.0)
-    textbook = np.array([[0.125 * np.prod(1 + O.RMAT_SIGNS[k] * O.NODE_SIGNS[j] / O.GS) for k in range(8)]
-                         for j in range(8)])
-    assert not np.allclose(W, textbook)
+    hand = _q3_by_hand()
+    assert np.allclose(W, hand, rtol=1e-14, atol=0)
+    gs = 0.57735026918962
+    assert np.isclose(W[0, 2], (1 + 1 / gs) ** 3 / 8, rtol=1e-14)
+    assert np.isclose(W[7, 4], (1 + 1 / gs) ** 3 / 8, rtol=1e-14)
+    assert np.isclose(W[7, 2], (1 - 1 / gs) ** 3 / 8, rtol=1e-14)
+    assert np.isclose(W[0, 0], (1 + 1 / gs) * (1 - 1 / gs) ** 2 / 8, rtol=1e-14)
+    # the plausible "fixes" differ from what the source computes
+    shape_corner = np.array([[0.125 * np.prod(1 + O.NODE_SIGNS[k] * O.NODE_SIGNS[j] / gs) for k in range(8)]
+                             for j in range(8)])
+    rmat_order = np.array([[0.125 * np.prod(1 + O.RMAT_SIGNS[k] * O.RMAT_SIGNS[j] / gs) for k in range(8)]
+                           for j in range(8)])
+    assert not np.allclose(hand, shape_corner) and not np.allclose(hand, rmat_order)
+    assert np.allclose(hand.sum(axis=1), 1.0)
+
+
+def test_extrapolation_one_element_probe():
+    """Q3 through nodal_stress on one element: CoElem = 1 at every node, so
+    node Elem[j] receives sum_k W[j, k] SVec[k], with SVec the Gauss-point
+    stresses.  A stress field that is a unit pulse at one Gauss point k is
+    reproduced as column k of the hand table (probed through the linear map
+    SVec -> nodal stress: the displacement is chosen so SVec is known)."""
+    g = mesh.structured_block(1, 1, 1, 1.0, 0.15, 4)
+    pp = O.pre_process(g.Node, g.Elem)
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(3 * 8) * 1e-3
+    en = O.element_nodes(g.Elem, pp["Nind"])
+    Coord = O.element_coords(g.Node, g.Elem, pp["Nind"])
+    SVec = O.gauss_stress(Coord, E0, NU, u.reshape(8, 3)[en])        # (2, 8, 1, 3)
+    S = O.nodal_stress(g.Node, g.Elem, pp, E0, NU, u)                 # (8, 6)
+    hand = _q3_by_hand()
+    sv = np.concatenate([SVec[0, :, 0, :], SVec[1, :, 0, :]], axis=1)  # (8 Gauss points, 6)
+    expect = np.zeros((8, 6))
+    expect[en[0]] = hand @ sv
+    assert np.allclose(S, expect, rtol=1e-12, atol=1e-12 * np.abs(sv).max())
+
+
+def test_extrapolation_unit_cube_by_hand():
+    """Q3 end to end with Gauss-point stresses derived by hand.  Unit cube
+    [0,1]^3, local node k at the corner with the signs of the shape-node
+    table (so x = (1 + xi)/2, y = (1 + eta)/2, z = (1 + zeta)/2), and
+    u = (x y z, 0, 0), which the trilinear space holds exactly.  Then
+    eps_xx = y z, gamma_xy = x z, gamma_xz = x y, and at the Gauss point g of
+    FEM_model_csr.py:408-415 (signs (-,-,-), (-,-,+), (-,+,-), ... : bit 2 of
+    g is xi, bit 1 eta, bit 0 zeta) sigma = ((l+2m) yz, l yz, l yz, m xz, 0,
+    m xy).  Node k must get sum_g hand[k, g] sigma_g: this pins the Gauss
+    order of SVec, the shear order and D together with the Q3 table."""
+    g = mesh.structured_block(1, 1, 1, 1.0, 0.0, 0)
+    pp = O.pre_process(g.Node, g.Elem)
+    x = g.Node[1:].T
+    u = np.zeros((8, 3))
+    u[:, 0] = x[:, 0] * x[:, 1] * x[:, 2]
+    S = O.get_stress(g.Node, g.Elem, pp, E0, NU, u.reshape(-1))
+    lam = E0 * NU / ((1 + NU) * (1 - 2 * NU))
+    mu = E0 / (2 * (1 + NU))
+    gs = 0.57735026918962
+    sig = np.zeros((8, 6))
+    for gp in range(8):
+        xi = [gs if (gp >> b) & 1 else -gs for b in (2, 1, 0)]
+        px, py, pz = [(1 + s) / 2 for s in xi]
+        sig[gp] = [(lam + 2 * mu) * py * pz, lam * py * pz, lam * py * pz, mu * px * pz, 0.0, mu * px * py]
+    en = O.element_nodes(g.Elem, pp["Nind"])[0]
+    expect = np.zeros((8, 6))
+    expect[en] = _q3_by_hand() @ sig
+    assert np.allclose(S[:, :6], expect, rtol=1e-11, atol=1e-9 * np.abs(sig).max())
+
+
+def test_von_mises_quirk_by_hand():
+    """Q4 (POST_process.py:93-94): sqrt((p0-p1)^2 + (p1-p2)^2 + (p2-p1)^2)
+    with no 1/2, on hand-computed principal triples: (3, 2, 0) -> sqrt(1 + 4 +
+    4) = 3 (true von Mises: sqrt(7)); (s, 0, 0) -> s (true: s); (5, 1, -1) ->
+    sqrt(16 + 4 + 4) = sqrt(24) (true: sqrt(28)).  The tensors are rotated
+    so the eigen-solve is exercised; principal values come out descending."""
+    c, s = np.cos(0.7), np.sin(0.7)
+    Rz = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+    c2, s2 = np.cos(-0.4), np.sin(-0.4)
+    Rx = np.array([[1, 0, 0], [0, c2, -s2], [0, s2, c2]])
+    R = Rz @ Rx
+    cases = [((3.0, 2.0, 0.0), 3.0), ((7.5, 0.0, 0.0), 7.5), ((5.0, 1.0, -1.0), np.sqrt(24.0)),
+             ((0.0, 2.0, 3.0), 3.0)]
+    vec = []
+    for p, _ in cases:
+        T = R @ np.diag(p) @ R.T
+        vec.append([T[0, 0], T[1, 1], T[2, 2], T[0, 1], T[1, 2], T[0, 2]])
+    principle, vm = O.principal_vm(np.array(vec))
+    for i, (p, v) in enumerate(cases):
+        assert np.allclose(principle[i], sorted(p, reverse=True), atol=1e-12)
+        assert np.isclose(vm[i, 0], v, rtol=1e-12)
+    assert not np.isclose(vm[0, 0], np.sqrt(7.0))
 
 
 # ---------------------------------------------------------------- Lanczos (a11)

@@ -55,7 +55,9 @@ def test_ascii_layout(tmp_path):
     T.write_dat(p, "part", xyz, fields, ["u", "v"], elem1)
     lines = p.read_text().splitlines()
     assert lines[0] == 'TITLE = "part.data"'
-    assert lines[1] == 'VARIABLES = "X [ m ]","Y [ m ]","Z [m]","u1" ,"v1" '
+    # POST_process.py:125, "%s " % ',"' + name + str(step + 1) + '"', evaluated by
+    # hand: ("%s " % ',"') is ',"' + ' ', then the name, the step and a quote
+    assert lines[1] == 'VARIABLES = "X [ m ]","Y [ m ]","Z [m]"," u1"," v1"'
     assert lines[2] == "ZONE  I=2, J=1, F=FEPOINT, ET=BRICK"
     assert lines[3] == "0.00000e+00 1.00000e+00 2.00000e+00 1.50000e+00 -2.00000e+00"
     assert lines[5] == "1 2 1 2 1 2 1 2"
