@@ -189,9 +189,12 @@ class Model:
     def Get_MatrixKN(self):
         """FEM_model_bsr.py:134-145: (KT, KM) as scipy CSR (3N x 3N)."""
         kt, km = self.assemble_KN("csr")
-        row, col = self._csr_index_K()
         n = self.Num_Free
-        return tuple(sp.csr_matrix((_to_host(v), col, row), shape=(n, n), dtype=np.float64) for v in (kt, km))
+        out = []
+        for v in (kt, km):
+            row, col = self._csr_index_K()          # each matrix its own index arrays
+            out.append(sp.csr_matrix((_to_host(v), col, row), shape=(n, n), dtype=np.float64))
+        return tuple(out)
 
     # ---- prestressed stiffness for modal analysis (no counterpart in the
     #      reference; Readme.md:189 lists stress stiffening and spin softening
