@@ -51,6 +51,7 @@ class FemMesh(ctypes.Structure):
         ("grp_elem_ptr", ctypes.c_void_p), ("grp_elems", ctypes.c_void_p),
         ("grp_enodes", ctypes.c_void_p),
         ("grp_task_ptr", ctypes.c_void_p), ("tasks", ctypes.c_void_p),
+        ("grp_row_rec", ctypes.c_void_p), ("grp_inc", ctypes.c_void_p),
     ]
 
 

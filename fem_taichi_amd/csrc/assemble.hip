@@ -42,14 +42,6 @@ struct AsmLds {
 
 __host__ __device__ inline int align16(int o) { return (o + 15) & ~15; }
 
-// Work-groups are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Give each XCD one contiguous run of the (spatially ordered) row
-// groups so that the coordinates neighbouring groups share stay in its L2.
-__device__ __forceinline__ int xcd_chunked(int b, int nb) {
-  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
-  return x * q + min(x, r) + i;
-}
-
 template <typename T>
 __host__ __device__ inline AsmLds asm_lds_layout(int eg, int rg, bool want_k, bool want_m) {
   AsmLds L;

@@ -16,7 +16,7 @@ void set_error(const char* fmt, ...) {
 
 extern "C" const char* fem_last_error(void) { return fem::g_err; }
 
-extern "C" int fem_abi_version(void) { return 1; }
+extern "C" int fem_abi_version(void) { return 2; }
 
 extern "C" int fem_device_count(int* count) {
   FEM_REQUIRE(count != nullptr, FEM_E_INVALID, "count is NULL");

@@ -46,6 +46,14 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Work-groups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Give each XCD one contiguous run of the (spatially ordered) row
+// groups so that the coordinates neighbouring groups share stay in its L2.
+__device__ __forceinline__ int xcd_chunked(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 // Gauss abscissa exactly as written in the reference (FEM_model_csr.py:93).
 constexpr double kGS = 0.57735026918962;
 

@@ -1,238 +1,452 @@
-// Load vector and stress recovery on the GPU.
+// Load vector and stress recovery on the GPU, row-owner and atomic-free.
 //
-// Both follow the same two-pass, atomic-free shape: a per-element kernel
-// writes each element's per-node contributions to scratch, and a per-node
-// kernel sums them over the node's incidence list (ascending element order,
-// the same list the pattern builder sorted), so every result is
-// deterministic.
+// One work-group owns one row group (a spatially compact set of node rows,
+// built once per mesh) and the elements touching its rows:
+//   phase 1: two threads per element, four Gauss points each.  The
+//            element's eight node coordinates (and displacements) are turned
+//            into Walsh coefficients
+//            c_S = sum_k (prod_{d in S} s_kd) v_k, S a subset of the three
+//            axes and s_k the shape-node signs, so that every derivative
+//            d v / d xi_d at a Gauss point is three fused multiply-adds of
+//            them.  The per-Gauss-point quantities go to the element's LDS
+//            slice, and after a barrier the thread pair carries them to the
+//            element's nodes by a tensor-product transform (three 2-point
+//            stages, one per axis); the per-node values of every element of
+//            the group stay in LDS.
+//   phase 2: one thread per row sums its incidences (the node -> element
+//            list, ascending element order) from LDS, in a fixed order,
+//            so the result is deterministic and no per-element values leave
+//            the chip.
+// Elements on a group's boundary are computed by every group they touch
+// (about 2x for 4 x 4 x 4 node bricks), which is cheaper than a round trip
+// of per-element node values through HBM.
 #include "common.hpp"
 
 namespace fem {
 namespace {
 
-// Element kernels: one lane per (element, Gauss point), 32 elements per
-// work-group.  The lane computes its Gauss point's quantities into LDS; after
-// a barrier lane (element, k) reduces over the 8 Gauss points for local node
-// k (in Gauss order), so no lane holds an element's 8 x 3 (or 8 x 6) outputs.
-constexpr int kFT = 256;
-constexpr int kFE = kFT / 8;
+constexpr int kFT = 256;   // threads per work-group: two per element
 
-// J at Gauss point g of the element with nodes nd, J[a][d] = dx_a/dxi_d
-// (the x_0 term of sum_k x_k (x) dN_k vanishes: sum_k dN_k = 0).
-__device__ __forceinline__ void jacobian_at(const fem_mesh& m, const int32_t nd[8], int g, double J[9], double R[3]) {
-  const int64_t n0 = nd[0];
-  const double x00 = m.coords[n0 * 3], x01 = m.coords[n0 * 3 + 1], x02 = m.coords[n0 * 3 + 2];
+// Walsh coefficient index S: bit 0 = xi, bit 1 = eta, bit 2 = zeta.
+// In-place fast transform of 8 values stored by node sign bits
+// (bit d set when s_kd = +1): afterwards w[S] = sum_b prod_{d in S} (+-1) w[b].
+__device__ __forceinline__ void walsh8(double w[8]) {
 #pragma unroll
-  for (int q = 0; q < 9; ++q) J[q] = 0.0;
-  R[0] = R[1] = R[2] = 0.0;
+  for (int d = 0; d < 3; ++d)
 #pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    const int64_t nk = nd[k];
-    const double x[3] = {m.coords[nk * 3], m.coords[nk * 3 + 1], m.coords[nk * 3 + 2]};
-    const double Nk = nshape(g, k);
-    const double dn[3] = {dnr(g, k, 0), dnr(g, k, 1), dnr(g, k, 2)};
-    const double x0[3] = {x00, x01, x02};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      R[a] = fma(Nk, x[a], R[a]);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) J[a * 3 + d] = fma(x[a] - x0[a], dn[d], J[a * 3 + d]);
-    }
-  }
+    for (int b = 0; b < 8; ++b)
+      if (!((b >> d) & 1)) {
+        const int b1 = b | (1 << d);
+        const double lo = w[b], hi = w[b1];
+        w[b] = lo + hi;
+        w[b1] = hi - lo;
+      }
 }
 
-__device__ __forceinline__ void element_nodes8(const fem_mesh& m, int64_t e, int32_t nd[8]) {
-  const int4* en = reinterpret_cast<const int4*>(m.elem + e * 8);
+// 8 * d v / d xi_d at the Gauss point with signs t0, t1, t2 (G = the
+// reference's abscissa), from the Walsh coefficients c of v.
+template <int D>
+__device__ __forceinline__ double dxi8(const double c[8], double g0, double g1, double g2, double g01,
+                                       double g02, double g12) {
+  // g_a = G t_a, g_ab = G^2 t_a t_b
+  if (D == 0) return fma(g12, c[7], fma(g2, c[5], fma(g1, c[3], c[1])));
+  if (D == 1) return fma(g02, c[7], fma(g2, c[6], fma(g0, c[3], c[2])));
+  return fma(g01, c[7], fma(g1, c[6], fma(g0, c[5], c[4])));
+}
+
+// One stage of a 2-point tensor transform along axis D over 8 values indexed
+// by sign bits: (v[b0], v[b1]) -> (a v0 + b v1, b v0 + a v1), b1 = b0 | 1 << D.
+template <int D>
+__device__ __forceinline__ void tstage(double v[8], double a, double b) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (!((q >> D) & 1)) {
+      const int q1 = q | (1 << D);
+      const double v0 = v[q], v1 = v[q1];
+      v[q] = fma(a, v0, b * v1);
+      v[q1] = fma(b, v0, a * v1);
+    }
+}
+
+__host__ __device__ constexpr int bitrev3(int j) { return ((j >> 2) & 1) | (j & 2) | ((j & 1) << 2); }
+
+struct GroupRange {
+  int e0, ne, r0, nr;
+};
+
+__device__ __forceinline__ GroupRange group_range(const fem_mesh& m, int g) {
+  GroupRange r;
+  r.e0 = m.grp_elem_ptr[g];
+  r.ne = m.grp_elem_ptr[g + 1] - r.e0;
+  r.r0 = m.grp_row_ptr[g];
+  r.nr = m.grp_row_ptr[g + 1] - r.r0;
+  return r;
+}
+
+__device__ __forceinline__ void group_element_nodes(const fem_mesh& m, int ge, int32_t nd[8]) {
+  const int4* en = reinterpret_cast<const int4*>(m.grp_enodes + (int64_t)ge * 8);
   const int4 a = en[0], b = en[1];
   nd[0] = a.x; nd[1] = a.y; nd[2] = a.z; nd[3] = a.w;
   nd[4] = b.x; nd[5] = b.y; nd[6] = b.z; nd[7] = b.w;
 }
 
-// InstallVectorF (FEM_model_csr.py:214-229) with UnitVectorF (:275-283):
-// Fe[k] = sum_g rho*detJ_g * N_gk * (|w|^2 R_g - (w.R_g) w), R_g = sum_k N_gk x_k.
+// Per-element LDS slice: NC components x 8 values, padded to an odd number
+// of doubles so neighbouring threads' slices start on different banks.
+template <int NC>
+__host__ __device__ constexpr int slice_stride() { return NC * 8 + 1; }
+
+// LDS of a group: the element slices, then the rows' 12-int records.
+template <int NC>
+__host__ __device__ inline int64_t fields_lds_bytes(int eg, int rg) {
+  return (int64_t)eg * slice_stride<NC>() * 8 + (int64_t)rg * 48;
+}
+
+// Row records (struct fem_mesh grp_row_rec) into LDS: issued first, so the
+// row phase's metadata arrives while phase 1 runs.
+__device__ __forceinline__ void load_row_record(const fem_mesh& m, int p, int4 rec[3]) {
+  const int4* rp = reinterpret_cast<const int4*>(m.grp_row_rec + (int64_t)p * 12);
+  rec[0] = rp[0];
+  rec[1] = rp[1];
+  rec[2] = rp[2];
+}
+
+// The LDS offset (in doubles) of incidence q of a row, from its record.
+template <int NC, bool STRESS>
+__device__ __forceinline__ int inc_offset(const fem_mesh& m, const int32_t* rec, int q) {
+  const int v = q < 8 ? rec[4 + q] : m.grp_inc[rec[3] + q - 8];
+  const int j = v & 7;
+  return (v >> 3) * slice_stride<NC>() + (STRESS ? bitrev3(j) : node_sign_bits(j));
+}
+
+// ---------------------------------------------------------------------------
+// Load vector: InstallVectorF (FEM_model_csr.py:214-229) with UnitVectorF
+// (:275-283), F_k = sum_g rho detJ_g N_k(xi_g) f(R_g), R_g = sum_m N_m(xi_g)
+// x_m, f(R) = |w|^2 R - (w.R) w; then LoadF (:205-210), the nodal loads.
+// N_k(xi_g) = prod_d (1 + s_kd t_gd G) / 2 is a tensor product, so the
+// Gauss-to-node step is three 2-point stages with a = (1 + G)/2 (same signs)
+// and b = (1 - G)/2.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kFT)
-body_force_elem(fem_mesh m, double rho, double w0, double w1, double w2, double* __restrict__ fe) {
-  __shared__ double s_d[kFE][8][4];   // rho*detJ, F(R) per (element, Gauss point)
-  const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
-  const int64_t e = (int64_t)blockIdx.x * kFE + le;
-  const bool valid = e < m.num_elem;
-  if (valid) {
-    int32_t nd[8];
-    element_nodes8(m, e, nd);
-    double J[9], R[3];
-    jacobian_at(m, nd, g, J, R);
-    s_d[le][g][0] = rho * det3(J);
-    s_d[le][g][1] = w1 * w1 * R[0] + w2 * w2 * R[0] - w0 * w1 * R[1] - w0 * w2 * R[2];
-    s_d[le][g][2] = w0 * w0 * R[1] + w2 * w2 * R[1] - w1 * w2 * R[2] - w1 * w0 * R[0];
-    s_d[le][g][3] = w1 * w1 * R[2] + w0 * w0 * R[2] - w2 * w0 * R[0] - w2 * w1 * R[1];
-  }
-  __syncthreads();
-  if (valid) {
-    const int k = g;   // this lane now owns local node k
-    double a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const double c = s_d[le][q][0] * nshape(q, k);
-      a0 = fma(c, s_d[le][q][1], a0);
-      a1 = fma(c, s_d[le][q][2], a1);
-      a2 = fma(c, s_d[le][q][3], a2);
+body_force_rows(fem_mesh m, double rho, double w0, double w1, double w2, const double* __restrict__ nodal,
+                double* __restrict__ f) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* s = reinterpret_cast<double*>(smem);
+  constexpr int SS = slice_stride<3>();
+  const GroupRange R = group_range(m, xcd_chunked(blockIdx.x, gridDim.x));
+  int32_t* s_rec = reinterpret_cast<int32_t*>(smem + (int64_t)R.ne * SS * 8);
+  const int tid = threadIdx.x;
+  const bool hr = tid < R.nr;       // rows <= kFT (checked on the host)
+  int4 rec[3];
+  if (hr) load_row_record(m, R.r0 + tid, rec);
+  const int le = tid >> 1, h = tid & 1;   // elements <= kFT / 2 (checked on the host)
+  const bool he = le < R.ne;
+  int32_t nd[8];
+  if (he) group_element_nodes(m, R.e0 + le, nd);
+  double nl0 = 0, nl1 = 0, nl2 = 0;
+  if (hr) {
+    reinterpret_cast<int4*>(s_rec + tid * 12)[0] = rec[0];
+    reinterpret_cast<int4*>(s_rec + tid * 12)[1] = rec[1];
+    reinterpret_cast<int4*>(s_rec + tid * 12)[2] = rec[2];
+    if (nodal) {
+      const int64_t i = rec[0].x;
+      nl0 = nodal[3 * i];
+      nl1 = nodal[3 * i + 1];
+      nl2 = nodal[3 * i + 2];
     }
-    fe[e * 24 + k * 3] = a0;
-    fe[e * 24 + k * 3 + 1] = a1;
-    fe[e * 24 + k * 3 + 2] = a2;
   }
-}
-
-// LoadF (FEM_model_csr.py:205-210) after the body force: per-node gather.
-__global__ void body_force_gather(fem_mesh m, const double* __restrict__ fe,
-                                  const double* __restrict__ nodal, double* __restrict__ f) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.num_node) return;
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int32_t t = m.ne_ptr[i]; t < m.ne_ptr[i + 1]; ++t) {
-    const int64_t c = (int64_t)m.ne_list[t] * 3;   // (e*8 + j) * 3
-    s0 += fe[c];
-    s1 += fe[c + 1];
-    s2 += fe[c + 2];
-  }
-  if (nodal) {
-    s0 += nodal[3 * i];
-    s1 += nodal[3 * i + 1];
-    s2 += nodal[3 * i + 2];
-  }
-  f[3 * i] = s0;
-  f[3 * i + 1] = s1;
-  f[3 * i + 2] = s2;
-}
-
-// UnitVectorS (POST_process.py:190-209) and the extrapolation of get_NdSol
-// (:172-181): per element, Gauss-point stresses, then per local node j
-// sum_g shapefunc_GS(GMat[j])[g] * SVec[g] -> se[e][j][6].
-__global__ void __launch_bounds__(kFT)
-stress_elem(fem_mesh m, const double* __restrict__ young, double young_scalar, double lhat, double mhat,
-            const double* __restrict__ u, double* __restrict__ se) {
-  __shared__ double s_s[kFE][8][6];   // Gauss-point stress (xx yy zz xy yz xz)
-  const int le = threadIdx.x >> 3, g = threadIdx.x & 7;
-  const int64_t e = (int64_t)blockIdx.x * kFE + le;
-  const bool valid = e < m.num_elem;
-  if (valid) {
-    int32_t nd[8];
-    element_nodes8(m, e, nd);
-    double J[9], R[3], Ji[9];
-    jacobian_at(m, nd, g, J, R);
-    inv3(J, Ji);
-    const double Ee = young ? young[e] : young_scalar;
-    const double lam = Ee * lhat, mu = Ee * mhat;
-    double exx = 0, eyy = 0, ezz = 0, gxy = 0, gyz = 0, gxz = 0;
+  const double ww = w0 * w0 + w1 * w1 + w2 * w2;
+  const double Gs = kGS;
+  if (he) {
+    double cx[3][8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const double dn[3] = {dnr(g, k, 0), dnr(g, k, 1), dnr(g, k, 2)};
-      double D[3];
+      const int b = node_sign_bits(k);
+      const int64_t n = nd[k];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) D[a] = dn[0] * Ji[a] + dn[1] * Ji[3 + a] + dn[2] * Ji[6 + a];
-      const int64_t nk = nd[k];
-      const double U0 = u[nk * 3], U1 = u[nk * 3 + 1], U2 = u[nk * 3 + 2];
-      exx = fma(D[0], U0, exx);
-      eyy = fma(D[1], U1, eyy);
-      ezz = fma(D[2], U2, ezz);
-      gxy += D[1] * U0 + D[0] * U1;
-      gyz += D[2] * U1 + D[1] * U2;
-      gxz += D[2] * U0 + D[0] * U2;
+      for (int a = 0; a < 3; ++a) cx[a][b] = m.coords[n * 3 + a];
     }
-    const double tr = lam * (exx + eyy + ezz);
-    s_s[le][g][0] = tr + 2 * mu * exx;
-    s_s[le][g][1] = tr + 2 * mu * eyy;
-    s_s[le][g][2] = tr + 2 * mu * ezz;
-    s_s[le][g][3] = mu * gxy;
-    s_s[le][g][4] = mu * gyz;
-    s_s[le][g][5] = mu * gxz;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) walsh8(cx[a]);
+    double* sl = s + le * SS;
+    const double g0 = h ? Gs : -Gs;   // this thread's Gauss points g = 4 h + gg
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const double g1 = Gs * gauss_sign(gg, 1), g2 = Gs * gauss_sign(gg, 2);
+      const double g01 = g0 * g1, g02 = g0 * g2, g12 = g1 * g2, g012 = g01 * g2;
+      double J[9];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        J[a * 3 + 0] = dxi8<0>(cx[a], g0, g1, g2, g01, g02, g12);
+        J[a * 3 + 1] = dxi8<1>(cx[a], g0, g1, g2, g01, g02, g12);
+        J[a * 3 + 2] = dxi8<2>(cx[a], g0, g1, g2, g01, g02, g12);
+      }
+      const double detJ = det3(J) * (1.0 / 512.0);
+      double Rg[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double* c = cx[a];
+        double v = fma(g0, c[1], c[0]);
+        v = fma(g1, c[2], v);
+        v = fma(g2, c[4], v);
+        v = fma(g01, c[3], v);
+        v = fma(g02, c[5], v);
+        v = fma(g12, c[6], v);
+        v = fma(g012, c[7], v);
+        Rg[a] = 0.125 * v;
+      }
+      const double wr = w0 * Rg[0] + w1 * Rg[1] + w2 * Rg[2];
+      const double sc = rho * detJ;
+      const int tb = bitrev3(gg) | h;   // Gauss sign bits of g (bit 0 = xi)
+      sl[0 * 8 + tb] = sc * (ww * Rg[0] - wr * w0);
+      sl[8 + tb] = sc * (ww * Rg[1] - wr * w1);
+      sl[16 + tb] = sc * (ww * Rg[2] - wr * w2);
+    }
   }
   __syncthreads();
-  if (valid) {
-    const int j = g;   // this lane now owns local node j
-    double out[6] = {0, 0, 0, 0, 0, 0};
+  // Gauss point -> node: the element's thread pair takes components h, h + 2 (of 3)
+  if (he) {
+    const double a = 0.5 * (1.0 + Gs), b = 0.5 * (1.0 - Gs);
+    double* sl = s + le * SS;
+    for (int c = h; c < 3; c += 2) {
+      double v[8];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      // W[j][q] = shapefunc_GS(GMat[j])[q]: GMat in Gauss-point sign order,
-      // weight index q taken with the shape-node sign table (as written).
-      double w = 0.125;
+      for (int q = 0; q < 8; ++q) v[q] = sl[c * 8 + q];
+      tstage<0>(v, a, b);
+      tstage<1>(v, a, b);
+      tstage<2>(v, a, b);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) w *= 1.0 + node_sign(q, a) * gauss_sign(j, a) / kGS;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) out[c] = fma(w, s_s[le][q][c], out[c]);
+      for (int q = 0; q < 8; ++q) sl[c * 8 + q] = v[q];   // by node sign bits
     }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) se[e * 48 + j * 6 + c] = out[c];
+  }
+  __syncthreads();
+  if (hr) {
+    const int32_t* rc = s_rec + tid * 12;
+    const int64_t i = rc[0];
+    const int cnt = rc[2];
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int q = 0; q < cnt; ++q) {
+      const double* sl = s + inc_offset<3, false>(m, rc, q);
+      s0 += sl[0];
+      s1 += sl[8];
+      s2 += sl[16];
+    }
+    f[3 * i] = s0 + nl0;
+    f[3 * i + 1] = s1 + nl1;
+    f[3 * i + 2] = s2 + nl2;
   }
 }
 
-// Symmetric 3x3 eigenvalues by cyclic Jacobi rotations (FP64), descending.
-__device__ void sym3_eigvals_desc(double a00, double a11, double a22, double a01, double a12,
-                                  double a02, double ev[3]) {
-  double A[3][3] = {{a00, a01, a02}, {a01, a11, a12}, {a02, a12, a22}};
+// Symmetric 3x3 eigenvalues by cyclic Jacobi rotations on the six distinct
+// entries (FP64), descending.  Rotation (p, q): t = tan(phi) is the smaller
+// root of t^2 + 2 t theta - 1 = 0, theta = (a_qq - a_pp) / (2 a_pq), written
+// as t = e sgn(d) / (|d| + hypot(d, e)) with d = a_qq - a_pp, e = 2 a_pq (one
+// square root, one division); then c = 1 / sqrt(1 + t^2), s = t c and the
+// full similarity update, which stays orthogonal however t rounds.
+__device__ __forceinline__ void jrot(double& app, double& aqq, double& apq, double& arp, double& arq) {
+  if (apq == 0.0) return;
+  const double d = aqq - app, e = 2.0 * apq;
+  const double den = fabs(d) + sqrt(fma(d, d, e * e));
+  const double t = (d >= 0.0 ? e : -e) / den;
+  const double c = rsqrt(fma(t, t, 1.0)), sn = t * c;
+  const double cc = c * c, ss = sn * sn, cs = c * sn;
+  const double pp = app, qq = aqq, pq = apq;
+  app = fma(cc, pp, fma(ss, qq, -2.0 * cs * pq));
+  aqq = fma(ss, pp, fma(cc, qq, 2.0 * cs * pq));
+  apq = fma(cs, pp - qq, (cc - ss) * pq);
+  const double rp = arp, rq = arq;
+  arp = fma(c, rp, -sn * rq);
+  arq = fma(sn, rp, c * rq);
+}
+
+__device__ void sym3_eigvals_desc(double a00, double a11, double a22, double a01, double a12, double a02,
+                                  double ev[3]) {
   for (int sweep = 0; sweep < 32; ++sweep) {
-    const double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
-    const double dg = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
-    if (!(off > 1e-36 * dg) ) break;   // also exits on NaN
-    for (int p = 0; p < 2; ++p) {
-      for (int q = p + 1; q < 3; ++q) {
-        const double apq = A[p][q];
-        if (apq == 0.0) continue;
-        const double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < 3; ++k) {   // rotate columns p, q
-          const double akp = A[k][p], akq = A[k][q];
-          A[k][p] = c * akp - s * akq;
-          A[k][q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < 3; ++k) {   // rotate rows p, q
-          const double apk = A[p][k], aqk = A[q][k];
-          A[p][k] = c * apk - s * aqk;
-          A[q][k] = s * apk + c * aqk;
-        }
-      }
-    }
+    const double off = a01 * a01 + a02 * a02 + a12 * a12;
+    const double dg = a00 * a00 + a11 * a11 + a22 * a22;
+    if (!(off > 1e-36 * dg)) break;   // also exits on NaN
+    jrot(a00, a11, a01, a02, a12);    // (p, q) = (0, 1), r = 2
+    jrot(a00, a22, a02, a01, a12);    // (0, 2), r = 1: a_r0 = a01, a_r2 = a12
+    jrot(a11, a22, a12, a01, a02);    // (1, 2), r = 0: a_r1 = a01, a_r2 = a02
   }
-  double v0 = A[0][0], v1 = A[1][1], v2 = A[2][2], t;
+  double v0 = a00, v1 = a11, v2 = a22, t;
   if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
   if (v1 < v2) { t = v1; v1 = v2; v2 = t; }
   if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
   ev[0] = v0; ev[1] = v1; ev[2] = v2;
 }
 
-// get_NdAve (POST_process.py:183-187) + Get_Stress host part (:79-95).
-__global__ void stress_gather(fem_mesh m, const double* __restrict__ se, double* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.num_node) return;
-  double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int32_t t = m.ne_ptr[i]; t < m.ne_ptr[i + 1]; ++t) {
-    const int64_t c = (int64_t)m.ne_list[t] * 6;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) s[q] += se[c + q];
+// ---------------------------------------------------------------------------
+// Stress recovery: UnitVectorS (POST_process.py:190-209) gives the stress at
+// Gauss point g (RMat order), get_NdSol (:172-181) carries it to element node
+// j with the weights shapefunc_GS(GMat[j])[g] = prod_d (1 + s_gd c_jd / G)/2
+// (s_g the shape-node signs of index g, c_j the RMat-order corner signs of
+// GMat[j]; the reference's order as written), get_NdAve (:183-187) divides
+// by CoElem, and the host part of Get_Stress (:79-95) the principal stresses
+// (descending) and the von Mises term sqrt((p0-p1)^2 + (p1-p2)^2 +
+// (p2-p1)^2) as written.
+//   grad u = H J^-1 with J = dx/dxi and H = du/dxi from the Walsh
+//   coefficients; with C the cofactors of J, grad u[a][p] = H[a].C[p] / det J.
+// The weights are a tensor product too: three stages with a = (1 + 1/G)/2
+// (same signs) and b = (1 - 1/G)/2.  The input slot of Gauss point g is the
+// sign bits of shape-node row g, the output slot of node j its RMat-order
+// corner (the bit reversal of j).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kFT, 3)
+stress_rows(fem_mesh m, const double* __restrict__ young, double young_scalar, double lhat, double mhat,
+            const double* __restrict__ u, double* __restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* s = reinterpret_cast<double*>(smem);
+  constexpr int SS = slice_stride<6>();
+  const GroupRange R = group_range(m, xcd_chunked(blockIdx.x, gridDim.x));
+  int32_t* s_rec = reinterpret_cast<int32_t*>(smem + (int64_t)R.ne * SS * 8);
+  const int tid = threadIdx.x;
+  const bool hr = tid < R.nr;       // rows <= kFT (checked on the host)
+  int4 rec[3];
+  if (hr) load_row_record(m, R.r0 + tid, rec);
+  const int le = tid >> 1, h = tid & 1;   // elements <= kFT / 2 (checked on the host)
+  const bool he = le < R.ne;
+  int32_t nd[8];
+  double Ee = young_scalar;
+  if (he) {
+    group_element_nodes(m, R.e0 + le, nd);
+    if (young) Ee = young[m.grp_elems[R.e0 + le]];
   }
-  const double ce = (double)m.co_elem[i];
+  if (hr) {
+    reinterpret_cast<int4*>(s_rec + tid * 12)[0] = rec[0];
+    reinterpret_cast<int4*>(s_rec + tid * 12)[1] = rec[1];
+    reinterpret_cast<int4*>(s_rec + tid * 12)[2] = rec[2];
+  }
+  const double Gs = kGS;
+  if (he) {
+    double cx[3][8], cu[3][8];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) s[q] = s[q] / ce;
+    for (int k = 0; k < 8; ++k) {
+      const int b = node_sign_bits(k);
+      const int64_t n = nd[k];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        cx[a][b] = m.coords[n * 3 + a];
+        cu[a][b] = u[n * 3 + a];
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      walsh8(cx[a]);
+      walsh8(cu[a]);
+    }
+    const double lam = Ee * lhat, mu = Ee * mhat;
+    double* sl = s + le * SS;
+    const double g0 = h ? Gs : -Gs;   // this thread's Gauss points g = 4 h + gg (RMat order)
+#pragma unroll
+    for (int gg = 0; gg < 4; ++gg) {
+      const double g1 = Gs * gauss_sign(gg, 1), g2 = Gs * gauss_sign(gg, 2);
+      const double g01 = g0 * g1, g02 = g0 * g2, g12 = g1 * g2;
+      double J[9];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        J[a * 3 + 0] = dxi8<0>(cx[a], g0, g1, g2, g01, g02, g12);
+        J[a * 3 + 1] = dxi8<1>(cx[a], g0, g1, g2, g01, g02, g12);
+        J[a * 3 + 2] = dxi8<2>(cx[a], g0, g1, g2, g01, g02, g12);
+      }
+      // cofactors C[p][d] of J (the 8x scale of J and H cancels in H C / det)
+      double C[9];
+      C[0] = J[4] * J[8] - J[5] * J[7];
+      C[1] = J[5] * J[6] - J[3] * J[8];
+      C[2] = J[3] * J[7] - J[4] * J[6];
+      C[3] = J[2] * J[7] - J[1] * J[8];
+      C[4] = J[0] * J[8] - J[2] * J[6];
+      C[5] = J[1] * J[6] - J[0] * J[7];
+      C[6] = J[1] * J[5] - J[2] * J[4];
+      C[7] = J[2] * J[3] - J[0] * J[5];
+      C[8] = J[0] * J[4] - J[1] * J[3];
+      const double rdet = 1.0 / (J[0] * C[0] + J[1] * C[1] + J[2] * C[2]);
+      double D[9];   // D[a][p] = H[a] . C[p] = det * du_a/dx_p, H = du/dxi one row at a time
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const double h0 = dxi8<0>(cu[a], g0, g1, g2, g01, g02, g12);
+        const double h1 = dxi8<1>(cu[a], g0, g1, g2, g01, g02, g12);
+        const double h2 = dxi8<2>(cu[a], g0, g1, g2, g01, g02, g12);
+#pragma unroll
+        for (int p = 0; p < 3; ++p) D[a * 3 + p] = fma(h2, C[p * 3 + 2], fma(h1, C[p * 3 + 1], h0 * C[p * 3]));
+      }
+      const double l2 = lam * rdet, m2 = mu * rdet;
+      const double tr = l2 * (D[0] + D[4] + D[8]);
+      const int vb = node_sign_bits(4 * h + gg);   // slot: sign bits of shape-node row g
+      sl[0 * 8 + vb] = fma(2.0 * m2, D[0], tr);
+      sl[1 * 8 + vb] = fma(2.0 * m2, D[4], tr);
+      sl[2 * 8 + vb] = fma(2.0 * m2, D[8], tr);
+      sl[3 * 8 + vb] = m2 * (D[1] + D[3]);   // xy
+      sl[4 * 8 + vb] = m2 * (D[5] + D[7]);   // yz
+      sl[5 * 8 + vb] = m2 * (D[2] + D[6]);   // xz
+    }
+  }
+  __syncthreads();
+  // Gauss point -> node: the element's thread pair takes components h, h + 2, h + 4
+  if (he) {
+    const double a = 0.5 * (1.0 + 1.0 / Gs), b = 0.5 * (1.0 - 1.0 / Gs);
+    double* sl = s + le * SS;
+#pragma unroll
+    for (int cc = 0; cc < 3; ++cc) {
+      const int c = 2 * cc + h;
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = sl[c * 8 + q];
+      tstage<0>(v, a, b);
+      tstage<1>(v, a, b);
+      tstage<2>(v, a, b);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sl[c * 8 + q] = v[q];   // by RMat-order corner bits
+    }
+  }
+  __syncthreads();
+  if (hr) {
+    const int32_t* rc = s_rec + tid * 12;
+    const int64_t i = rc[0];
+    const double ce = (double)rc[1];
+    const int cnt = rc[2];
+    double sv[6] = {0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < cnt; ++q) {
+      const double* sl = s + inc_offset<6, true>(m, rc, q);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sv[c] += sl[c * 8];
+    }
+    double* o = out + i * 10;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[c] = sv[c] / ce;
+  }
+}
+
+// Principal stresses (descending) and the reference's von Mises term, one
+// thread per node, from the averaged stress columns 0..5 of out.
+__global__ void stress_principal(int64_t n, double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double* o = out + i * 10;
   double ev[3];
-  sym3_eigvals_desc(s[0], s[1], s[2], s[3], s[4], s[5], ev);
+  sym3_eigvals_desc(o[0], o[1], o[2], o[3], o[4], o[5], ev);
   const double vm = sqrt((ev[0] - ev[1]) * (ev[0] - ev[1]) + (ev[1] - ev[2]) * (ev[1] - ev[2]) +
                          (ev[2] - ev[1]) * (ev[2] - ev[1]));
-  double* o = out + i * 10;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) o[q] = s[q];
   o[6] = ev[0];
   o[7] = ev[1];
   o[8] = ev[2];
   o[9] = vm;
 }
 
-int check_fields_mesh(const fem_mesh* m) {
+template <int NC>
+int check_fields_mesh(const fem_mesh* m, int64_t* lds) {
   FEM_REQUIRE(m != nullptr, FEM_E_INVALID, "mesh is NULL");
   FEM_REQUIRE(m->num_node >= 0 && m->num_elem >= 0, FEM_E_INVALID, "negative size");
-  FEM_REQUIRE(m->num_node == 0 || (m->ne_ptr && m->co_elem), FEM_E_INVALID, "incidence missing");
-  FEM_REQUIRE(m->num_elem == 0 || (m->coords && m->elem && m->ne_list), FEM_E_INVALID,
-              "mesh arrays missing");
+  FEM_REQUIRE(m->num_groups == 0 || (m->grp_row_ptr && m->grp_elem_ptr && m->grp_row_rec), FEM_E_INVALID,
+              "row groups or row records (grp_row_rec) missing");
+  FEM_REQUIRE(m->num_elem == 0 || (m->coords && m->grp_enodes && m->grp_elems), FEM_E_INVALID,
+              "mesh arrays missing (coords, group elements)");
+  FEM_REQUIRE(((uintptr_t)m->grp_enodes & 15) == 0 && ((uintptr_t)m->grp_row_rec & 15) == 0, FEM_E_INVALID,
+              "grp_enodes and grp_row_rec must be 16-byte aligned");
+  FEM_REQUIRE(m->max_group_elems >= 0 && m->max_group_elems <= kFT / 2, FEM_E_LIMIT,
+              "row groups of %d elements: at most %d", m->max_group_elems, kFT / 2);
+  FEM_REQUIRE(m->max_group_rows >= 0 && m->max_group_rows <= kFT, FEM_E_LIMIT,
+              "row groups of %d rows: at most %d", m->max_group_rows, kFT);
+  *lds = fields_lds_bytes<NC>(m->max_group_elems, m->max_group_rows);
   return FEM_OK;
 }
 
@@ -241,53 +455,50 @@ int check_fields_mesh(const fem_mesh* m) {
 
 using namespace fem;
 
-extern "C" int64_t fem_body_force_work_bytes(int64_t num_elem) { return 24 * 8 * num_elem + 256; }
-extern "C" int64_t fem_stress_work_bytes(int64_t num_elem) { return 48 * 8 * num_elem + 256; }
+extern "C" int64_t fem_body_force_work_bytes(int64_t num_elem) { return num_elem < 0 ? -1 : 0; }
+extern "C" int64_t fem_stress_work_bytes(int64_t num_elem) { return num_elem < 0 ? -1 : 0; }
 
 extern "C" int fem_body_force(const fem_mesh* mesh, double rho, const double* omega3,
                               const double* nodal_load, double* f_out, void* work,
                               int64_t work_bytes, void* stream) {
-  int rc = check_fields_mesh(mesh);
+  (void)work;
+  int64_t lds = 0;
+  int rc = check_fields_mesh<3>(mesh, &lds);
   if (rc) return rc;
   FEM_REQUIRE(omega3 != nullptr, FEM_E_INVALID, "omega3 is NULL (host pointer to 3 doubles)");
   FEM_REQUIRE(mesh->num_node == 0 || f_out, FEM_E_INVALID, "f_out is NULL");
-  FEM_REQUIRE(work_bytes >= fem_body_force_work_bytes(mesh->num_elem), FEM_E_INVALID, "work too small");
-  FEM_REQUIRE(((uintptr_t)mesh->elem & 15) == 0, FEM_E_INVALID, "elem must be 16-byte aligned");
+  FEM_REQUIRE(work_bytes >= 0, FEM_E_INVALID, "negative work_bytes");
+  if (mesh->num_node == 0 || mesh->num_groups == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
-  double* fe = reinterpret_cast<double*>(work);
-  if (mesh->num_elem > 0) {
-    body_force_elem<<<ceil_div(mesh->num_elem, kFE), kFT, 0, st>>>(*mesh, rho, omega3[0], omega3[1],
-                                                                    omega3[2], fe);
-    FEM_LAUNCH_CHECK();
-  }
-  if (mesh->num_node > 0) {
-    body_force_gather<<<ceil_div(mesh->num_node, 256), 256, 0, st>>>(*mesh, fe, nodal_load, f_out);
-    FEM_LAUNCH_CHECK();
-  }
+  FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(body_force_rows),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(body_force_rows, dim3((unsigned)mesh->num_groups), dim3(kFT), (size_t)lds, st, *mesh, rho,
+                     omega3[0], omega3[1], omega3[2], nodal_load, f_out);
+  FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
 
 extern "C" int fem_stress_recover(const fem_mesh* mesh, const double* young, double young_scalar,
                                   double nu, const double* u, double* out, void* work,
                                   int64_t work_bytes, void* stream) {
-  int rc = check_fields_mesh(mesh);
+  (void)work;
+  int64_t lds = 0;
+  int rc = check_fields_mesh<6>(mesh, &lds);
   if (rc) return rc;
   FEM_REQUIRE(mesh->num_node == 0 || (u && out), FEM_E_INVALID, "u/out NULL");
-  FEM_REQUIRE(work_bytes >= fem_stress_work_bytes(mesh->num_elem), FEM_E_INVALID, "work too small");
-  FEM_REQUIRE(((uintptr_t)mesh->elem & 15) == 0, FEM_E_INVALID, "elem must be 16-byte aligned");
+  FEM_REQUIRE(work_bytes >= 0, FEM_E_INVALID, "negative work_bytes");
   FEM_REQUIRE(nu > -1.0 && nu < 0.5, FEM_E_INVALID, "Poisson ratio %g outside (-1, 0.5)", nu);
+  if (mesh->num_node == 0 || mesh->num_groups == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
-  double* se = reinterpret_cast<double*>(work);
   const double lhat = nu / ((1 + nu) * (1 - 2 * nu));
   const double mhat = 1.0 / (2 * (1 + nu));
-  if (mesh->num_elem > 0) {
-    stress_elem<<<ceil_div(mesh->num_elem, kFE), kFT, 0, st>>>(*mesh, young, young_scalar, lhat, mhat,
-                                                              u, se);
-    FEM_LAUNCH_CHECK();
-  }
-  if (mesh->num_node > 0) {
-    stress_gather<<<ceil_div(mesh->num_node, 256), 256, 0, st>>>(*mesh, se, out);
-    FEM_LAUNCH_CHECK();
-  }
+  FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(stress_rows),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(stress_rows, dim3((unsigned)mesh->num_groups), dim3(kFT), (size_t)lds, st, *mesh, young,
+                     young_scalar, lhat, mhat, u, out);
+  FEM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(stress_principal, dim3((unsigned)ceil_div(mesh->num_node, 256)), dim3(256), 0, st,
+                     mesh->num_node, out);
+  FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

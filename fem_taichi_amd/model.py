@@ -147,11 +147,9 @@ class Model:
         """The load vector (3N,) f64 on the device."""
         if "F" not in self._cache:
             f = torch.empty(3 * self.Num_Node, dtype=torch.float64, device=device())
-            wb = _lib.lib().fem_body_force_work_bytes(self.Num_Elem)
-            work = torch.empty(wb, dtype=torch.uint8, device=device())
             om = (ctypes.c_double * 3)(*[float(w) for w in self.OM])
-            _lib.call("fem_body_force", self.dev.ref, float(self.DE), om, self.nodal_load.data_ptr(),
-                      f.data_ptr(), work.data_ptr(), wb, stream_ptr())
+            _lib.call("fem_body_force", self.Physics.fields_dev.ref, float(self.DE), om,
+                      self.nodal_load.data_ptr(), f.data_ptr(), None, 0, stream_ptr())
             self._cache["F"] = f
         return self._cache["F"]
 

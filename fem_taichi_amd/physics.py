@@ -28,6 +28,13 @@ MAX_GROUP_ELEMS = 48
 KN_BRICK = (2, 2, 1)
 KN_MORTON_ROWS = 4
 KN_MAX_GROUP_ELEMS = 24
+# Load vector and stress recovery (one thread per element, 392 B of LDS per
+# element): 4 x 4 x 4 node bricks touch 125 elements, so each element is
+# computed by about 2 groups instead of about 3 with the assembly's bricks;
+# 64 rows of the Morton order otherwise, at most 128 elements.
+FIELD_BRICK = (4, 4, 4)
+FIELD_MORTON_ROWS = 64
+FIELD_MAX_GROUP_ELEMS = 128
 
 
 class DeviceMesh:
@@ -42,6 +49,8 @@ class DeviceMesh:
             self.t[k] = v
         for k in ("grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "grp_enodes", "grp_task_ptr", "tasks"):
             self.t[k] = torch.as_tensor(groups[k], dtype=torch.int32).contiguous().to(dev)
+        for k in ("grp_row_rec", "grp_inc"):      # the field kernels' row records (field groups only)
+            self.t[k] = torch.as_tensor(groups.get(k, np.zeros(0, np.int32)), dtype=torch.int32).contiguous().to(dev)
         self.num_node = int(coords.shape[0])
         self.num_elem = int(elem.shape[0])
         self.nnzb = int(pattern["re_node"].numel())
@@ -49,7 +58,7 @@ class DeviceMesh:
         m.num_node, m.num_elem, m.nnzb = self.num_node, self.num_elem, self.nnzb
         for name in ("coords", "elem", "sm_node", "re_node", "co_node", "co_elem", "ne_ptr", "ne_list",
                      "grp_row_ptr", "grp_rows", "grp_elem_ptr", "grp_elems", "grp_enodes", "grp_task_ptr",
-                     "tasks"):
+                     "tasks", "grp_row_rec", "grp_inc"):
             setattr(m, name, self.t[name].data_ptr() if self.t[name].numel() else None)
         m.num_groups = int(groups["num_groups"])
         m.max_group_elems = int(groups["max_group_elems"])
@@ -89,6 +98,39 @@ def gpu_pattern(elem_idx: torch.Tensor, N: int):
               ne_list.data_ptr(), sm_node.data_ptr(), re_node.data_ptr(), st)
     return dict(co_elem=co_elem, ne_ptr=ne_ptr, ne_list=ne_list[:8 * E], co_node=co_node,
                 sm_node=sm_node, re_node=re_node[:nnzb])
+
+
+def row_records(groups, ne_ptr, ne_list, co_elem):
+    """The row records of the row-owner load-vector and stress kernels
+    (struct fem_mesh grp_row_rec / grp_inc): per position p of grp_rows, 12
+    int32 = [node i, CoElem[i], incidence count c, overflow offset o, first 8
+    incidences (local element << 3 | local node), -1 padded]; incidences 8..c-1
+    of a row go to the overflow list at o.  Incidences keep the ascending
+    element order of ne_list."""
+    rows = np.asarray(groups["grp_rows"], dtype=np.int64)
+    N = len(rows)
+    ne_ptr = np.asarray(ne_ptr, dtype=np.int64)
+    cnt = np.diff(ne_ptr)
+    inc_val = (np.asarray(groups["ne_local"], dtype=np.int64) << 3) | (np.asarray(ne_list, dtype=np.int64) & 7)
+    rec = np.full((N, 12), -1, dtype=np.int32)
+    c = cnt[rows]
+    rec[:, 0] = rows
+    rec[:, 1] = np.asarray(co_elem)[rows]
+    rec[:, 2] = c
+    ovf_n = np.maximum(c - 8, 0)
+    ovf_off = np.concatenate([[0], np.cumsum(ovf_n)[:-1]]) if N else np.zeros(0, np.int64)
+    rec[:, 3] = ovf_off
+    start = ne_ptr[rows]
+    for q in range(8):
+        has = c > q
+        rec[has, 4 + q] = inc_val[start[has] + q]
+    total = int(ovf_n.sum())
+    inc = np.zeros(total, dtype=np.int32)
+    if total:
+        p_idx = np.repeat(np.arange(N), ovf_n)
+        k = np.arange(total) - np.repeat(ovf_off, ovf_n) + 8
+        inc[:] = inc_val[start[p_idx] + k]
+    return rec.reshape(-1), inc
 
 
 class Physics:
@@ -181,7 +223,26 @@ class Physics:
         self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
         self._symbolic = (coords, pat, ne_ptr, ne_list)
         self._kn_dev = None
+        self._fields_dev = None
         return self
+
+    @property
+    def fields_dev(self):
+        """The same mesh and pattern with the larger row groups of the
+        load-vector and stress kernels (no task lists; built on first use)."""
+        if self._fields_dev is None:
+            coords, pat, ne_ptr, ne_list = self._symbolic
+            gid = node_group_ids(self.Num_Node, node_dims=self.node_dims, xyz=coords, brick=FIELD_BRICK,
+                                 rows=FIELD_MORTON_ROWS)
+            groups = build_row_groups(self.Num_Node, self.Num_Elem, ne_ptr, ne_list, gid, FIELD_MAX_GROUP_ELEMS,
+                                      co_node=self.CoNode)
+            groups["grp_task_ptr"] = np.zeros(int(groups["num_groups"]) + 1, dtype=np.int32)
+            groups["tasks"] = np.zeros(0, dtype=np.int32)
+            groups["max_group_tasks"] = 0
+            groups["grp_enodes"] = np.ascontiguousarray(self.elem_nodes[groups["grp_elems"]].reshape(-1))
+            groups["grp_row_rec"], groups["grp_inc"] = row_records(groups, ne_ptr, ne_list, self.CoElem)
+            self._fields_dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
+        return self._fields_dev
 
     @property
     def kn_dev(self):

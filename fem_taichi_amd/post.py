@@ -45,11 +45,9 @@ class Post:
         ex = np.asarray(self.EX, dtype=np.float64)
         young = None if ex.ndim == 0 else torch.as_tensor(ex).to(device())
         out = torch.empty(N * 10, dtype=torch.float64, device=device())
-        wb = _lib.lib().fem_stress_work_bytes(E)
-        work = torch.empty(wb, dtype=torch.uint8, device=device())
-        _lib.call("fem_stress_recover", self.Physics.dev.ref, young.data_ptr() if young is not None else None,
+        _lib.call("fem_stress_recover", self.Physics.fields_dev.ref, young.data_ptr() if young is not None else None,
                   float(ex) if ex.ndim == 0 else 0.0, float(self.PO), self._u.data_ptr(), out.data_ptr(),
-                  work.data_ptr(), wb, stream_ptr())
+                  None, 0, stream_ptr())
         return out.cpu().numpy().reshape(N, 10)
 
     # ---- Tecplot output (POST_process.py:32-69, 101-135) ---------------------

@@ -95,6 +95,14 @@ typedef struct fem_mesh {
   const int32_t* grp_enodes;   /* 8 per entry of grp_elems, 16-byte aligned */
   const int32_t* grp_task_ptr; /* G + 1 */
   const int32_t* tasks;        /* 2 int32 per task, 8-byte aligned */
+  const int32_t* grp_row_rec;  /* row records of the row-owner load-vector and stress
+                                  kernels, 12 int32 per position p of grp_rows, 16-byte
+                                  aligned: node i = grp_rows[p], CoElem[i], its incidence
+                                  count c, an offset o into grp_inc, then its first 8
+                                  incidences as (index of the element within the group's
+                                  element list) << 3 | local node, ascending element;
+                                  incidences 8 .. c-1 are grp_inc[o .. o + c - 8) */
+  const int32_t* grp_inc;      /* overflow incidences of rows with more than 8 */
 } fem_mesh;
 
 /* ---- a1: symbolic pattern (replaces READ_physics.py:42-61 pre_process) --- */
@@ -188,6 +196,11 @@ int fem_apply_penalty(const fem_mesh* mesh, const int32_t* bound_node, const dou
  *      LoadF :205-210) -------------------------------------------------------
  * f_out (3N) = centrifugal body force of every element gathered per node in
  * element order + nodal_load (3N, nullable).  work: 24*E doubles. */
+/* Row-owner kernels on the mesh's row groups (one work-group per group, the
+ * group's element values kept in LDS); they need grp_row_rec and no scratch:
+ * fem_body_force_work_bytes / fem_stress_work_bytes return 0 (the work
+ * arguments are kept for the ABI and ignored).  LDS: 200 B (load vector) or
+ * 392 B (stress) per element of the largest group, at most 160 KB. */
 int64_t fem_body_force_work_bytes(int64_t num_elem);
 int fem_body_force(const fem_mesh* mesh, double rho, const double* omega3,
                    const double* nodal_load, double* f_out, void* work, int64_t work_bytes,
@@ -196,7 +209,7 @@ int fem_body_force(const fem_mesh* mesh, double rho, const double* omega3,
 /* ---- a13: stress recovery (replaces get_NdSol/get_NdAve/UnitVectorS
  *      POST_process.py:172-209 and the host eigh/von Mises :79-95) ---------
  * u (3N) displacements; out (N x 10): [sxx syy szz txy tyz txz, p1 p2 p3
- * (descending), the reference's von Mises term].  work: 48*E doubles. */
+ * (descending), the reference's von Mises term]. */
 int64_t fem_stress_work_bytes(int64_t num_elem);
 int fem_stress_recover(const fem_mesh* mesh, const double* young, double young_scalar,
                        double nu, const double* u, double* out, void* work,
