@@ -118,6 +118,7 @@ SIGNATURES = {
     "fem_dense_gemm_nb_f32": (_I32, [_P, _I64, _P, _P, _I64, _P]),
     "fem_gemm_tn_work_bytes": (_I64, [_I64, _I64]),
     "fem_gemm_tn_f64": (_I32, [_P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P]),
+    "fem_gemm_tn_f64_noalloc": (_I32, [_P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P]),
     "fem_fcg_work_bytes": (_I64, [_I64]),
     "fem_fcg_direction": (_I32, [_I64, _I64, _P, _I32, _P, _P, _P, _I32, _P, _I64, _P]),
     "fem_fcg_update": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),

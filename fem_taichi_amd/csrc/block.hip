@@ -8,6 +8,8 @@
 // index (thread id mod nb) and sums every column in a fixed order.  As in the
 // single-vector PCG, each of kGrid blocks writes per-column partials and the
 // consuming kernel re-sums them in index order: no atomics, bit-reproducible.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace fem {
@@ -526,17 +528,11 @@ extern "C" int fem_mpcg_iterate(const fem_mesh* mesh, const double* vals, const 
 
 extern "C" int64_t fem_gemm_tn_work_bytes(int64_t p, int64_t q) { return 8 * (int64_t)kGridTN * p * q + 256; }
 
-extern "C" int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p,
-                               int64_t q, double* C, void* work, int64_t work_bytes, void* stream) {
-  FEM_REQUIRE(A && B && C && work, FEM_E_INVALID, "null pointer");
-  FEM_REQUIRE(n >= 0 && p >= 1 && q >= 1 && p <= 512 && q <= 64 && p * q <= 32 * kTT, FEM_E_INVALID,
-              "need 1 <= p <= 512, 1 <= q <= 64, p*q <= %d (got %lld x %lld)", 32 * kTT, (long long)p,
-              (long long)q);
-  FEM_REQUIRE(lda >= p && ldb >= q, FEM_E_INVALID, "row strides lda %lld < p or ldb %lld < q", (long long)lda,
-              (long long)ldb);
-  FEM_REQUIRE(work_bytes >= fem_gemm_tn_work_bytes(p, q), FEM_E_INVALID, "work too small");
-  hipStream_t st = as_stream(stream);
-  double* part = reinterpret_cast<double*>(work);
+namespace fem {
+namespace {
+
+int gemm_tn_impl(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
+                 double* C, double* part, hipStream_t st) {
   const int pi = (int)p, qi = (int)q, pqi = (int)(p * q);
   const int G = pqi <= kTT / 2 ? kTT / pqi : 1;
   const int nacc = G > 1 ? 1 : (pqi + kTT - 1) / kTT;
@@ -557,6 +553,67 @@ extern "C" int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, in
   gemm_tn_sum_kernel<<<pqi, kTT, 0, st>>>(part, pqi, nblk, C);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
+}
+
+int check_gemm_tn(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
+                  const double* C) {
+  FEM_REQUIRE(A && B && C, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(n >= 0 && p >= 1 && q >= 1 && p <= 512 && q <= 64 && p * q <= 32 * kTT, FEM_E_INVALID,
+              "need 1 <= p <= 512, 1 <= q <= 64, p*q <= %d (got %lld x %lld)", 32 * kTT, (long long)p,
+              (long long)q);
+  FEM_REQUIRE(lda >= p && ldb >= q, FEM_E_INVALID, "row strides lda %lld < p or ldb %lld < q", (long long)lda,
+              (long long)ldb);
+  return FEM_OK;
+}
+
+// The no-allocation variant's scratch: the partial blocks of at most
+// kNoAllocPQ outputs, kGridTN x kNoAllocPQ doubles (16 MiB), one copy per GPU
+// for the life of the process.  Its device address is looked up once per
+// device (hipGetSymbolAddress with that device current) and kept here; host
+// code passes only that address to kernels, never the array's name.
+constexpr int kNoAllocPQ = FEM_GEMM_TN_NOALLOC_MAX_PQ;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) double scratch[(size_t)kGridTN * kNoAllocPQ];
+double* g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_for_current_device(double** out) {
+  int dev = 0;
+  FEM_CHECK_HIP(hipGetDevice(&dev));
+  FEM_REQUIRE(dev >= 0 && dev < kMaxDevices, FEM_E_LIMIT, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    FEM_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = reinterpret_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return FEM_OK;
+}
+
+}  // namespace
+}  // namespace fem
+
+extern "C" int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p,
+                               int64_t q, double* C, void* work, int64_t work_bytes, void* stream) {
+  int rc = check_gemm_tn(A, lda, B, ldb, n, p, q, C);
+  if (rc) return rc;
+  FEM_REQUIRE(work, FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE(work_bytes >= fem_gemm_tn_work_bytes(p, q), FEM_E_INVALID, "work too small");
+  return gemm_tn_impl(A, lda, B, ldb, n, p, q, C, reinterpret_cast<double*>(work), as_stream(stream));
+}
+
+extern "C" int fem_gemm_tn_f64_noalloc(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n,
+                                       int64_t p, int64_t q, double* C, void* stream) {
+  int rc = check_gemm_tn(A, lda, B, ldb, n, p, q, C);
+  if (rc) return rc;
+  FEM_REQUIRE(p * q <= kNoAllocPQ, FEM_E_LIMIT,
+              "p*q = %lld exceeds the %d outputs of the static scratch (use fem_gemm_tn_f64 with a workspace)",
+              (long long)(p * q), kNoAllocPQ);
+  double* part = nullptr;
+  rc = scratch_for_current_device(&part);
+  if (rc) return rc;
+  return gemm_tn_impl(A, lda, B, ldb, n, p, q, C, part, as_stream(stream));
 }
 
 // ---- FCG(1) vector steps -------------------------------------------------

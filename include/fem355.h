@@ -326,6 +326,22 @@ int64_t fem_gemm_tn_work_bytes(int64_t p, int64_t q);
 int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
                     double* C, void* work, int64_t work_bytes, void* stream);
 
+/* fem_gemm_tn_f64 without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The same kernels
+ * and launch shape as fem_gemm_tn_f64, so C is bit-identical to it; the
+ * partials live in a static device array of FEM_GEMM_TN_NOALLOC_MAX_PQ x 1024
+ * doubles (16 MiB), held by each GPU the library runs on until the process
+ * ends, and its address is looked up on the first call on each device.  The
+ * call allocates nothing.  p * q above FEM_GEMM_TN_NOALLOC_MAX_PQ gives
+ * FEM_E_LIMIT.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams, other host threads) race and must be serialised by
+ * the caller; consecutive calls on one stream are safe (each call writes every
+ * partial it reads). */
+#define FEM_GEMM_TN_NOALLOC_MAX_PQ 2048
+int fem_gemm_tn_f64_noalloc(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p,
+                            int64_t q, double* C, void* stream);
+
 /* Flexible CG, FCG(1), vector steps on nb column-interleaved vectors (n rows,
  * entry (i, c) at i*nb + c), around an outside preconditioner (the multigrid
  * V-cycle).  The reference's static and shift-invert solves are direct
