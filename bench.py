@@ -12,9 +12,14 @@ The JSON line also carries:
   roofline      -- the assembly kernel's algorithmic bytes per launch
                    (24 N + 32 E + 8 E + 72 nnzb) over its mean launch time
                    (HIP events on the launch stream), against 8 TB/s;
-  cpu_baseline  -- the C/OpenMP restatement of the reference's CPU assembly
-                   (oracle/assemble_ref.c) on the same mesh, median of 5 after
-                   one warm-up run, on this host's cores;
+  cpu_baseline  -- the reference's CPU algorithms restated and timed on this
+                   host's cores in a child process (oracle/cpu_legs.py): the
+                   C/OpenMP assembly (oracle/assemble_ref.c, -march=native) on
+                   the same mesh; the direct static solve (MKL Pardiso, else
+                   SuperLU) and the NumPy block Lanczos on the 20x20x50
+                   cantilever, beside the GPU forms on that same cantilever;
+  end_to_end    -- the reference API's Get_MatrixKL() wall time on the slab,
+                   including the D2H copies into fresh host arrays;
   f32           -- the same assembly into an FP32 BSR copy;
   static_solve  -- seconds to an FP64-verified 1e-8 relative residual on the
                    rotating twisted blade 400x50x50 (1,043,001 nodes), per
@@ -85,26 +90,61 @@ def load_traffic(key):
         return json.load(f).get(key)
 
 
-def cpu_baseline(grid, phys):
-    import numpy as np
-    from oracle import c_oracle
-    threads = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    xyz = np.ascontiguousarray(grid.Node[1:].T)
-    en = phys.elem_nodes
-    ex = np.asarray(grid.EX)
-    args = (xyz, en, ex if ex.ndim else None, float(ex) if ex.ndim == 0 else 0.0, grid.PO,
-            phys.SmNode, phys.CoNode, phys.ReNode)
-    c_oracle.assemble_K_csr(*args, with_cols=False)          # warm-up (like excluding Taichi JIT)
-    times = []
-    while len(times) < 5 or sum(times) < 10.0:     # about 10 s of CPU work, at least 5 runs
-        t = time.perf_counter()
-        c_oracle.assemble_K_csr(*args, with_cols=False)
-        times.append(time.perf_counter() - t)
-    med = statistics.median(times)
-    return {"value": grid.Num_Elem / med, "unit": "hex8 elems assembled/s", "cores": int(os.environ["OMP_NUM_THREADS"]),
-            "kind": "port", "sample": f"full workload ({grid.Num_Elem} elements), FP64 CSR K, median of "
-                                      f"{len(times)} runs ({med:.3f} s each, {sum(times):.1f} s in all) after 1 warm-up"}
+def cpu_legs(timeout=600):
+    """The CPU baseline (oracle/cpu_legs.py) in a child process started
+    before this process touches the GPU: MKL Pardiso cannot share a process
+    with the OpenMP runtime torch loads.  Returns its JSON (or an error)."""
+    import subprocess
+    print("# cpu baseline: oracle/cpu_legs.py (assembly, static solve, block Lanczos)", flush=True)
+    try:
+        r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_legs.py")], capture_output=True,
+                           text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"error": f"cpu_legs.py did not finish in {timeout} s"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        return {"error": f"cpu_legs.py exit {r.returncode}: {r.stderr[-500:]}"}
+    return json.loads(lines[-1])
+
+
+def cpu_baseline_record(legs):
+    if "error" in legs:
+        return {"value": None, "unit": "hex8 elems assembled/s", "cores": None, "kind": "port", "sample": legs["error"]}
+    a = legs["assembly"]
+    return {"value": a["value"], "unit": a["unit"], "cores": legs["host"]["threads"], "kind": "port",
+            "sample": f"{a['workload']}: the C/OpenMP restatement of the reference's CPU assembly "
+                      f"(oracle/assemble_ref.c, {a['build']}), median of {a['runs']} full runs "
+                      f"({a['seconds_median']:.3f} s each) after 1 warm-up",
+            "host_cpu": legs["host"]["cpu"], "static": legs["static"], "modal": legs["modal"]}
+
+
+def time_c0_gpu():
+    """The CPU legs' problems on the GPU: the 20x20x50 cantilever's static
+    solve (multigrid FCG to an FP64-verified 1e-8, and plain FP64 PCG) and
+    its 20 lowest modes with block 8, second of two runs each."""
+    import torch
+    from fem_taichi_amd import Eigen_solver, Model, Physics, mesh, multigrid, solvers
+    g = mesh.cantilever_20x20x50()
+    phys = Physics.from_grid(g)
+    model = Model(g.name, phys, g.Materials, Omegas=(0.0, 0.0, 0.0))
+    F = model.F_device()
+    K64 = solvers.BsrOperator(model.dev, model.K_bsr("f64"))
+    multigrid.hierarchy_for(model)
+    out = {"workload": "cantilever_20x20x50 (67473 DOF)"}
+    for key, fn in (("static_mg", lambda: multigrid.solve_mg(model, K64, F, tol=1e-8)[1]),
+                    ("static_pcg_fp64_bj", lambda: solvers.solve_pcg64(K64, F, tol=1e-8)[1])):
+        fn()
+        info = fn()
+        out[key] = {"seconds": info.seconds, "iterations": info.iterations, "rel_residual": info.rel_residual}
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hz, _ = Eigen_solver(model).Solve_cpu(num_eigen=20, num_block=8, seed=0)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+    out["modal_20_modes_block_8"] = {"seconds": secs, "freq_hz_first": [float(v) for v in hz[:4]]}
+    print(f"# c0 gpu {json.dumps(out)}", flush=True)
+    return out
 
 
 def time_solves(mesh_spec, all_forms=False):
@@ -233,6 +273,7 @@ def main():
         print("bench.py runs on one GPU only: there is no multi-GPU path (the assembly would shard by node "
               "slabs with one halo element layer, the solvers by row blocks with halo exchanges).", file=sys.stderr)
         sys.exit(2)
+    legs = None if args.no_cpu_baseline else cpu_legs()       # before anything touches the GPU
     import numpy as np
     import torch
     from fem_taichi_amd import Model, Physics, mesh
@@ -297,8 +338,22 @@ def main():
         "f32": {"ms_per_assembly": f32_ms, "elems_per_s": E / (f32_ms * 1e-3),
                 "alg_GBs": alg_bytes_K(N, E, nnzb, True, 4) / (f32_ms * 1e-3) / 1e9},
     }
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(g, phys)
+    # end to end: the reference API's Get_MatrixKL() (assembly, penalty, D2H of the values and index
+    # arrays into fresh host arrays, scipy wrapper), warm (second and later calls)
+    e2e = []
+    for _ in range(4):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        Kh = model.Get_MatrixKL()
+        e2e.append(time.perf_counter() - t)
+        del Kh
+    result["end_to_end"] = {"Get_MatrixKL_seconds": statistics.median(e2e[1:]), "first_call_seconds": e2e[0],
+                            "includes": "assembly + penalty + D2H of values and CSR index arrays + scipy csr_matrix",
+                            "workload": "slab_60x60x55"}
+    if legs is not None:
+        result["cpu_baseline"] = cpu_baseline_record(legs)
+        if not args.no_solve:
+            result["cpu_baseline"]["gpu_same_problems"] = time_c0_gpu()
     if not args.no_solve:
         result["static_solve"] = time_solves(args.solve_mesh, all_forms=args.solve_all)
     if args.modal:
