@@ -44,7 +44,7 @@ def test_fem_mesh_struct_matches_header():
     text = open(HEADER).read()
     body = re.search(r"typedef struct fem_mesh \{(.*?)\} fem_mesh;", text, flags=re.S).group(1)
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
-    fields = re.findall(r"(\w+)\s*;", body)
+    fields = re.findall(r"(\w+)\s*(?:\[\d+\])?\s*;", body)
     assert fields == [f[0] for f in _lib.FemMesh._fields_]
 
 
