@@ -20,6 +20,8 @@ The JSON line also carries:
                    cantilever, beside the GPU forms on that same cantilever;
   end_to_end    -- the reference API's Get_MatrixKL() wall time on the slab,
                    including the D2H copies into fresh host arrays;
+  fields        -- the load vector and stress recovery kernels on the slab,
+                   time per launch and roofline fraction;
   f32           -- the same assembly into an FP32 BSR copy;
   static_solve  -- seconds to an FP64-verified 1e-8 relative residual on the
                    rotating twisted blade 400x50x50 (1,043,001 nodes), per
@@ -116,6 +118,48 @@ def cpu_baseline_record(legs):
                       f"(oracle/assemble_ref.c, {a['build']}), median of {a['runs']} full runs "
                       f"({a['seconds_median']:.3f} s each) after 1 warm-up",
             "host_cpu": legs["host"]["cpu"], "static": legs["static"], "modal": legs["modal"]}
+
+
+def time_fields(g, model, phys, reps=20):
+    """The row-owner load vector (fem_body_force) and stress recovery
+    (fem_stress_recover, principal stresses and von Mises included) on the
+    workload mesh: mean launch time by HIP events on the launch stream and the
+    roofline fraction of their algorithmic bytes (load vector 24N + 32E + 48N:
+    coordinates, connectivity, nodal loads in, F out; stress 48N + 40E + 80N:
+    coordinates and U in, connectivity and modulus, 10 values per node out)."""
+    import ctypes
+    import torch
+    from fem_taichi_amd import _lib
+    from fem_taichi_amd.runtime import stream_ptr
+    N, E = g.Num_Node, g.Num_Elem
+    fd = phys.fields_dev
+    st = stream_ptr()
+    f = torch.empty(3 * N, dtype=torch.float64, device="cuda")
+    om = (ctypes.c_double * 3)(0.0, 0.0, 100.0)
+    u = torch.randn(3 * N, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
+    out = torch.empty(10 * N, dtype=torch.float64, device="cuda")
+    yp = model.young.data_ptr() if model.young is not None else None
+    calls = {
+        "load_vector": (lambda: _lib.call("fem_body_force", fd.ref, float(g.DE), om, model.nodal_load.data_ptr(),
+                                          f.data_ptr(), None, 0, st), 24 * N + 32 * E + 48 * N),
+        "stress": (lambda: _lib.call("fem_stress_recover", fd.ref, yp, model.young_scalar, float(g.PO),
+                                     u.data_ptr(), out.data_ptr(), None, 0, st), 48 * N + 40 * E + 80 * N),
+    }
+    res = {}
+    for key, (fn, ab) in calls.items():
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gbs = ab / (ms * 1e-3) / 1e9
+        res[key] = {"us_per_launch": ms * 1e3, "alg_bytes": ab, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return res
 
 
 def time_c0_gpu():
@@ -338,6 +382,7 @@ def main():
         "f32": {"ms_per_assembly": f32_ms, "elems_per_s": E / (f32_ms * 1e-3),
                 "alg_GBs": alg_bytes_K(N, E, nnzb, True, 4) / (f32_ms * 1e-3) / 1e9},
     }
+    result["fields"] = time_fields(g, model, phys)
     # end to end: the reference API's Get_MatrixKL() (assembly, penalty, D2H of the values and index
     # arrays into fresh host arrays, scipy wrapper), warm (second and later calls)
     e2e = []
