@@ -109,6 +109,10 @@ SIGNATURES = {
     "fem_colsum": (_I32, [_P, _I32, _I64, _I32, _I32, _P, _P, _I64, _P]),
     "fem_relu_bwd_bf16": (_I32, [_P, _P, _I64, _P, _P]),
     "fem_to_bf16": (_I32, [_P, _I64, _I32, _I32, _P, _P]),
+    "fem_meshconv_fwd_c4_bf16": (_I32, [_P, _P, _P, _I32, _I32, _I32, _I32, _P, _P, _I32, _P]),
+    "fem_meshconv_pack_c4_bf16": (_I32, [_P, _I32, _I32, _P, _P]),
+    "fem_meshconv_features_c4": (_I32, [_P, _I64, _P, _P, _P, _P]),
+    "fem_meshconv_correction": (_I32, [_P, _I32, _I64, _P, _D, _P, _P, _P]),
     "fem_pcg_replace_residual": (_I32, [_MP, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
     "fem_mg_smooth": (_I32, [_MP, _P, _I32, _P, ctypes.c_float, _P, _P, _P, _I32, _P]),
     "fem_mg_restrict": (_I32, [_TP, _P, _P, _I64, _P]),

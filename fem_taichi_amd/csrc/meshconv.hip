@@ -186,6 +186,124 @@ conv3_fwd_tiled_kernel(const bf16* __restrict__ x, int cin, const bf16* __restri
   }
 }
 
+// ---- first layer with its 4 input channels packed along K: k = 4 tap + ci
+// (108 of 128 used), so a 16-node tile takes 4 MFMA k-steps instead of the
+// padded kernel's 27 taps of 32 channels (6.75x fewer MFMAs, and 8 bytes of
+// input per node instead of 64).  x4 is (P, 4) bf16; wp[s][co][kk] (4 x cout
+// x 32 bf16) holds w[tap][co][ci] at 32 s + kk = 4 tap + ci, zero beyond tap
+// 26.  Same tile as the padded kernel (16 x 4 x 4 nodes, wave w = z-plane w):
+// lane (m, kq) of k-step s supplies the 4 channels of taps 8 s + 2 kq and
+// 8 s + 2 kq + 1 at node m, read as two 8-byte halo words from LDS; the B
+// fragments of all four k-steps stay in registers.
+constexpr int kLdsC4 = kTileNodes * 64 * 2;             // bf16 output staging (the halo fits below it)
+static_assert(kHalo * 8 <= kLdsC4, "the packed halo must fit in the output staging area");
+
+template <int NT>
+__global__ void __launch_bounds__(kCT)
+conv3_fwd_c4_kernel(const bf16* __restrict__ x4, const bf16* __restrict__ wp, const float* __restrict__ bias,
+                    int cout, int nx, int ny, int nz, bf16* __restrict__ y_act, bf16* __restrict__ y_pre,
+                    int relu) {
+  __shared__ __align__(16) unsigned char smem[kLdsC4];
+  uint2* sH = reinterpret_cast<uint2*>(smem);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int bx = blockIdx.x * kTX, by = blockIdx.y * kTY, bz = blockIdx.z * kTZ;
+  for (int hn = tid; hn < kHalo; hn += kCT) {
+    const int hx = hn % kHX, hy = (hn / kHX) % kHY, hz = hn / (kHX * kHY);
+    const int gx = bx + hx - 1, gy = by + hy - 1, gz = bz + hz - 1;
+    uint2 v = make_uint2(0u, 0u);
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+      v = *reinterpret_cast<const uint2*>(x4 + (((int64_t)gz * ny + gy) * nx + gx) * 4);
+    sH[hn] = v;
+  }
+  bf16x8 Bf[4][NT];
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      Bf[s][t] = *reinterpret_cast<const bf16x8*>(wp + ((int64_t)s * cout + t * 16 + m) * 32 + kq * 8);
+  f32x4 acc[kTY][NT];
+#pragma unroll
+  for (int r = 0; r < kTY; ++r)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int ta = 8 * s + 2 * kq, tb = ta + 1;
+    const int dza = ta / 9 - 1, dya = (ta / 3) % 3 - 1, dxa = ta % 3 - 1;
+    const int dzb = tb / 9 - 1, dyb = (tb / 3) % 3 - 1, dxb = tb % 3 - 1;
+#pragma unroll
+    for (int r = 0; r < kTY; ++r) {
+      uint2 va = make_uint2(0u, 0u), vb = make_uint2(0u, 0u);
+      if (ta < 27) va = sH[((wave + 1 + dza) * kHY + (r + 1 + dya)) * kHX + (m + 1 + dxa)];
+      if (tb < 27) vb = sH[((wave + 1 + dzb) * kHY + (r + 1 + dyb)) * kHX + (m + 1 + dxb)];
+      const uint4 a4 = make_uint4(va.x, va.y, vb.x, vb.y);
+      bf16x8 Af;
+      __builtin_memcpy(&Af, &a4, sizeof(Af));
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Af, Bf[s][t], acc[r][t], 0, 0, 0);
+    }
+  }
+  // epilogue: bias, stage the tile (node-major, cout contiguous) in LDS, write node rows
+  __syncthreads();
+  bf16* sOb = reinterpret_cast<bf16*>(smem);
+#pragma unroll
+  for (int r = 0; r < kTY; ++r)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int ch = t * 16 + m;
+      const float b = bias ? bias[ch] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int node = (kq * 4 + i) + kTX * (r + kTY * wave);
+        sOb[node * cout + ch] = (bf16)(acc[r][t][i] + b);
+      }
+    }
+  __syncthreads();
+  for (int e = tid; e < kTileNodes * cout; e += kCT) {
+    const int node = e / cout, ch = e - node * cout;
+    const int tx = node % kTX, ty = (node / kTX) % kTY, tz = node / (kTX * kTY);
+    const int gx = bx + tx, gy = by + ty, gz = bz + tz;
+    if (gx >= nx || gy >= ny || gz >= nz) continue;
+    const int64_t o = (((int64_t)gz * ny + gy) * nx + gx) * cout + ch;
+    const bf16 v = sOb[e];
+    if (y_pre) y_pre[o] = v;
+    if (y_act) y_act[o] = (relu && (float)v < 0.f) ? (bf16)0.0f : v;
+  }
+}
+
+// wp[s][co][kk] = w[tap][co][ci] with 32 s + kk = 4 tap + ci, tap < 27, ci < 4 (else 0)
+__global__ void pack_c4_kernel(const bf16* __restrict__ w, int cin_w, int cout, bf16* __restrict__ wp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * cout * 32) return;
+  const int s = t / (cout * 32), co = (t / 32) % cout, kk = t % 32;
+  const int k = 32 * s + kk, tap = k >> 2, ci = k & 3;
+  wp[t] = tap < 27 ? w[((int64_t)tap * cout + co) * cin_w + ci] : (bf16)0.0f;
+}
+
+// x4[p][c] = bf16(float(r[3p + c] / s)) for c < 3, bf16(efeat[p]) for c = 3; s = *scale
+__global__ void features_c4_kernel(const double* __restrict__ r, int64_t P, const double* __restrict__ scale,
+                                   const float* __restrict__ efeat, bf16* __restrict__ x4) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * P) return;
+  const int64_t p = t >> 2;
+  const int c = (int)(t & 3);
+  x4[t] = c < 3 ? (bf16)(float)(r[3 * p + c] / scale[0]) : (bf16)efeat[p];
+}
+
+// c[3p + j] = (double)y[p * ld + j] * (s / e_ref) * mask[3p + j] (mask may be NULL); s = *scale
+__global__ void correction_kernel(const float* __restrict__ y, int ld, int64_t P, const double* __restrict__ scale,
+                                  double e_ref, const double* __restrict__ mask, double* __restrict__ c) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * P) return;
+  const int64_t p = t / 3;
+  const int j = (int)(t - 3 * p);
+  const double v = (double)y[p * ld + j] * (scale[0] / e_ref);
+  c[t] = mask ? v * mask[t] : v;
+}
+
 // wt[tap][ci][co] = w[26 - tap][co][ci]   (backward-data weights)
 __global__ void conv3_flip_kernel(const bf16* __restrict__ w, int cout, int cin, bf16* __restrict__ wt,
                                   int cout_t, int cin_t) {
@@ -401,6 +519,58 @@ extern "C" int fem_to_bf16(const float* a, int64_t P, int cs, int cd, void* out,
   FEM_REQUIRE(a && out && P >= 0 && cs > 0 && cd >= cs, FEM_E_INVALID, "bad arguments");
   if (P == 0) return FEM_OK;
   to_bf16_kernel<<<ceil_div(P * cd, 256), 256, 0, as_stream(stream)>>>(a, P, cs, cd, (bf16*)out);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_meshconv_fwd_c4_bf16(const void* x4, const void* wp, const float* bias, int cout, int nx, int ny,
+                                        int nz, void* y_act, void* y_pre, int relu, void* stream) {
+  FEM_REQUIRE(cout > 0 && cout % 16 == 0 && cout <= 64, FEM_E_INVALID, "cout must be a multiple of 16 <= 64");
+  FEM_REQUIRE(nx > 0 && ny > 0 && nz > 0, FEM_E_INVALID, "bad grid");
+  FEM_REQUIRE(x4 && wp && (y_act || y_pre), FEM_E_INVALID, "null pointer");
+  FEM_REQUIRE((reinterpret_cast<uintptr_t>(x4) & 7) == 0 && (reinterpret_cast<uintptr_t>(wp) & 15) == 0,
+              FEM_E_INVALID, "x4 must be 8-byte and wp 16-byte aligned");
+  const dim3 grid((unsigned)ceil_div(nx, kTX), (unsigned)ceil_div(ny, kTY), (unsigned)ceil_div(nz, kTZ));
+  FEM_REQUIRE(grid.y <= 65535 && grid.z <= 65535, FEM_E_LIMIT, "grid too large for the tiled convolution");
+  hipStream_t st = as_stream(stream);
+  const bf16* xb = (const bf16*)x4;
+  const bf16* wb = (const bf16*)wp;
+  bf16 *ya = (bf16*)y_act, *yp = (bf16*)y_pre;
+  switch (cout / 16) {
+    case 1: conv3_fwd_c4_kernel<1><<<grid, kCT, 0, st>>>(xb, wb, bias, cout, nx, ny, nz, ya, yp, relu); break;
+    case 2: conv3_fwd_c4_kernel<2><<<grid, kCT, 0, st>>>(xb, wb, bias, cout, nx, ny, nz, ya, yp, relu); break;
+    case 3: conv3_fwd_c4_kernel<3><<<grid, kCT, 0, st>>>(xb, wb, bias, cout, nx, ny, nz, ya, yp, relu); break;
+    default: conv3_fwd_c4_kernel<4><<<grid, kCT, 0, st>>>(xb, wb, bias, cout, nx, ny, nz, ya, yp, relu); break;
+  }
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_meshconv_pack_c4_bf16(const void* w, int cin_w, int cout, void* wp, void* stream) {
+  FEM_REQUIRE(w && wp && cin_w >= 4 && cout > 0 && cout % 16 == 0 && cout <= 64, FEM_E_INVALID,
+              "bad packing arguments");
+  const int n = 4 * cout * 32;
+  pack_c4_kernel<<<ceil_div(n, 256), 256, 0, as_stream(stream)>>>((const bf16*)w, cin_w, cout, (bf16*)wp);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_meshconv_features_c4(const double* r, int64_t P, const double* scale, const float* efeat,
+                                        void* x4, void* stream) {
+  FEM_REQUIRE(P >= 0, FEM_E_INVALID, "negative node count");
+  if (P == 0) return FEM_OK;
+  FEM_REQUIRE(r && scale && efeat && x4, FEM_E_INVALID, "null pointer");
+  features_c4_kernel<<<ceil_div(4 * P, 256), 256, 0, as_stream(stream)>>>(r, P, scale, efeat, (bf16*)x4);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_meshconv_correction(const float* y, int ld, int64_t P, const double* scale, double e_ref,
+                                       const double* mask, double* c, void* stream) {
+  FEM_REQUIRE(P >= 0 && ld >= 3, FEM_E_INVALID, "bad size");
+  if (P == 0) return FEM_OK;
+  FEM_REQUIRE(y && scale && c, FEM_E_INVALID, "null pointer");
+  correction_kernel<<<ceil_div(3 * P, 256), 256, 0, as_stream(stream)>>>(y, ld, P, scale, e_ref, mask, c);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

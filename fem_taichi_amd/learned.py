@@ -88,6 +88,37 @@ class MeshConvNet:
         for i in range(3):
             self.w[i] = sd[f"w{i}"].to(d).float()
             self.b[i] = sd[f"b{i}"].to(d).float()
+        self._infer = None
+
+    def _infer_weights(self):
+        """BF16 weights of the inference pass (the first layer packed for
+        fem_meshconv_fwd_c4_bf16), rebuilt when a weight tensor changes."""
+        key = tuple((id(w), w._version) for w in self.w)
+        cache = getattr(self, "_infer", None)
+        if cache is None or cache[0] != key:
+            wb = [_bf(w) for w in self.w]
+            wp = torch.empty(4, HID, 32, dtype=torch.bfloat16, device=wb[0].device)
+            _lib.call("fem_meshconv_pack_c4_bf16", wb[0].data_ptr(), CIN0, HID, wp.data_ptr(), stream_ptr())
+            self._infer = cache = (key, wp, wb[1], wb[2])
+        return cache[1:]
+
+    def forward_infer(self, x4, grid: ConvWork):
+        """x4: (P, 4) bf16 input -> (P, 16) f32 output (3 channels used); the
+        first layer runs packed (4 MFMA k-steps per tile instead of 27)."""
+        st = stream_ptr()
+        P = grid.P
+        d = device()
+        wp, w1, w2 = self._infer_weights()
+        a1 = torch.empty(P, HID, dtype=torch.bfloat16, device=d)
+        _lib.call("fem_meshconv_fwd_c4_bf16", x4.data_ptr(), wp.data_ptr(), self.b[0].data_ptr(), HID, grid.nx,
+                  grid.ny, grid.nz, a1.data_ptr(), None, 1, st)
+        a2 = torch.empty(P, HID, dtype=torch.bfloat16, device=d)
+        _lib.call("fem_meshconv_fwd_bf16", a1.data_ptr(), HID, w1.data_ptr(), self.b[1].data_ptr(), HID, grid.nx,
+                  grid.ny, grid.nz, a2.data_ptr(), None, None, 1, st)
+        y = torch.empty(P, COUT, dtype=torch.float32, device=d)
+        _lib.call("fem_meshconv_fwd_bf16", a2.data_ptr(), HID, w2.data_ptr(), self.b[2].data_ptr(), COUT, grid.nx,
+                  grid.ny, grid.nz, None, None, y.data_ptr(), 0, st)
+        return y
 
     # ------------------------------------------------------------- passes
     def forward(self, x0, grid: ConvWork, keep=False):
@@ -225,6 +256,7 @@ class LearnedPreconditioner:
         self.gain_patience = gain_patience
         self.low = 0
         self.net_off = False
+        self._pending = None           # device gain of the last application, read at the next one
 
     def features(self, r):
         P = self.grid.P
@@ -244,6 +276,7 @@ class LearnedPreconditioner:
             if not self.use_net:
                 return z
             return z + self.correction(r)
+        self._read_gain()
         z = self.base(r)
         if not self.use_net or self.net_off:
             return z
@@ -255,22 +288,44 @@ class LearnedPreconditioner:
         self.taus.append(tau)
         if self.min_gain is not None:
             # the error energy the correction removes, tau (c.r_a), against the
-            # energy r.z of the V-cycle's own step; after ``gain_patience``
+            # energy r.z of the V-cycle's own step; it is read at the next
+            # application (no host wait here), and after ``gain_patience``
             # applications in a row below ``min_gain`` the network is switched
             # off for the rest of the solve (the V-cycle alone from then on)
-            gain = float(torch.nan_to_num(tau * cra / torch.dot(r, z), nan=0.0))
-            self.gains.append(gain)
-            self.low = self.low + 1 if gain < self.min_gain else 0
-            if self.low >= self.gain_patience:
-                self.net_off = True
+            self._pending = torch.nan_to_num(tau * cra / torch.dot(r, z), nan=0.0)
         return z + tau * c
 
+    def _read_gain(self):
+        if self._pending is None:
+            return
+        gain = float(self._pending)
+        self._pending = None
+        self.gains.append(gain)
+        self.low = self.low + 1 if gain < self.min_gain else 0
+        if self.low >= self.gain_patience:
+            self.net_off = True
+
     def correction(self, r, keep=False):
-        x0, s = self.features(r)
-        y = self.net.forward(x0, self.grid, keep=keep)
-        c = y[:, :3].reshape(-1).double() * (s / E_REF)
-        if self.mask is not None:
-            c = c * self.mask
+        """The masked correction c = s * net(r / s, log E) / E_REF, s = RMS(r).
+        keep=False (inference): the packed first layer and fused input and
+        output kernels; keep=True (training): the padded pass that keeps the
+        activations for the backward kernels."""
+        if keep:
+            x0, s = self.features(r)
+            y = self.net.forward(x0, self.grid, keep=True)
+            c = y[:, :3].reshape(-1).double() * (s / E_REF)
+            if self.mask is not None:
+                c = c * self.mask
+            return c
+        st = stream_ptr()
+        P = self.grid.P
+        s = torch.sqrt(torch.mean(r * r)).clamp_min(1e-300)
+        x4 = torch.empty(P, 4, dtype=torch.bfloat16, device=r.device)
+        _lib.call("fem_meshconv_features_c4", r.data_ptr(), P, s.data_ptr(), self.efeat.data_ptr(), x4.data_ptr(), st)
+        y = self.net.forward_infer(x4, self.grid)
+        c = torch.empty(3 * P, dtype=torch.float64, device=r.device)
+        _lib.call("fem_meshconv_correction", y.data_ptr(), COUT, P, s.data_ptr(), E_REF,
+                  self.mask.data_ptr() if self.mask is not None else None, c.data_ptr(), st)
         return c
 
 
@@ -406,6 +461,7 @@ def solve_learned(model, K64, F, tol=1e-8, net=None, weights=None, maxiter=10000
         pre = LearnedPreconditioner(net, phys.node_dims, efeat, dinv64, free_mask=free_dof_mask(model),
                                     use_net=use_net, base=mg, K=K64, min_gain=min_gain, gain_patience=gain_patience)
         x, info = multigrid.solve_fcg_mg(model, K64, F, pre, tol=tol, maxiter=min(maxiter, 1000), **kw)
+        pre._read_gain()
         info["fell_back"] = False
         info["applications"] = pre.applications
         info["net_applications"] = len(pre.taus)

@@ -392,6 +392,25 @@ int fem_colsum(const void* a, int is_bf16, int64_t P, int C, int chunk, float* o
 int fem_relu_bwd_bf16(const void* da, const void* z, int64_t n, void* dz, void* stream);
 /* out (P x cd, bf16) = a (P x cs, f32), channels cs..cd-1 zero. */
 int fem_to_bf16(const float* a, int64_t P, int cs, int cd, void* out, void* stream);
+/* The first layer with 4 input channels, packed along K (k = 4*tap + ci, 4
+ * MFMA k-steps per tile instead of 27): x4 (P x 4 bf16, 8-byte aligned), wp
+ * from fem_meshconv_pack_c4_bf16 (16-byte aligned); writes y_act (ReLU when
+ * relu != 0) and/or y_pre, bf16.  Same function as fem_meshconv_fwd_bf16 on
+ * the input padded to 32 channels, to FP32 summation order. */
+int fem_meshconv_fwd_c4_bf16(const void* x4, const void* wp, const float* bias, int cout, int nx, int ny,
+                             int nz, void* y_act, void* y_pre, int relu, void* stream);
+/* wp[s][co][kk] (4 x cout x 32 bf16) = w[tap][co][ci] at 32*s + kk = 4*tap + ci
+ * (tap < 27, ci < 4; zero elsewhere), from w[27][cout][cin_w]. */
+int fem_meshconv_pack_c4_bf16(const void* w, int cin_w, int cout, void* wp, void* stream);
+/* x4[p][c] = bf16(float(r[3p+c] / *scale)) for c < 3 and bf16(efeat[p]) for
+ * c = 3: the network input from a residual r (3P doubles) and the per-node
+ * feature efeat (P floats); scale is a device pointer to one double. */
+int fem_meshconv_features_c4(const double* r, int64_t P, const double* scale, const float* efeat,
+                             void* x4, void* stream);
+/* c[3p+j] = (double)y[p*ld + j] * (*scale / e_ref) * mask[3p+j] (mask NULL: 1),
+ * j < 3: the correction from the network output y (P x ld floats). */
+int fem_meshconv_correction(const float* y, int ld, int64_t P, const double* scale, double e_ref,
+                            const double* mask, double* c, void* stream);
 
 #ifdef __cplusplus
 }

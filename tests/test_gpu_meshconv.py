@@ -72,3 +72,61 @@ def test_deterministic(gpu):
     assert torch.equal(y1, y2)
     for a, b in zip(g1[0] + g1[1], g2[0] + g2[1]):
         assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("dims", [(9, 7, 5), (33, 6, 9)])
+def test_packed_first_layer(gpu, dims):
+    """The inference pass (first layer packed along K from the (P, 4) input)
+    against the FP64 restatement, like the padded pass, and against the
+    padded pass itself (FP32 accumulation order only)."""
+    from fem_taichi_amd.learned import MeshConvNet, ConvWork
+    nx, ny, nz = dims
+    print(f"\ncase packed first layer grid {dims}", flush=True)
+    g = ConvWork(nx, ny, nz)
+    net = MeshConvNet(seed=6)
+    for b in net.b:
+        b.normal_(0, 0.1)
+    net.b[2][3:] = 0
+    x4 = torch.randn(g.P, 4, generator=torch.Generator().manual_seed(7)).to(torch.bfloat16)
+    x = torch.zeros(g.P, 32, dtype=torch.bfloat16)
+    x[:, :4] = x4
+    y_pad = net.forward(x.cuda(), g)
+    y = net.forward_infer(x4.cuda().contiguous(), g)
+    ye, _, _ = R.torch_reference_forward(net, x.cuda(), g, emulate_bf16=True)
+    yr, _, _ = R.torch_reference_forward(net, x.cuda(), g)
+    errs = (rel(y[:, :3], ye[:, :3]), rel(y[:, :3], yr[:, :3]), rel(y[:, :3], y_pad[:, :3]))
+    print("packed vs emulated, exact, padded:", errs, flush=True)
+    assert errs[0] < EMU_TOL and errs[1] < TOL and errs[2] < EMU_TOL
+    assert y[:, 3:].abs().max().item() == 0.0
+    # the packed weights follow the weights: a change of w0 is seen
+    y1 = net.forward_infer(x4.cuda().contiguous(), g)
+    assert torch.equal(y, y1)
+    net.w[0].mul_(0.5)
+    y2 = net.forward_infer(x4.cuda().contiguous(), g)
+    assert not torch.equal(y1, y2)
+
+
+def test_features_and_correction_kernels(gpu):
+    """fem_meshconv_features_c4 / fem_meshconv_correction against the torch
+    expressions they replace (bit-identical)."""
+    from fem_taichi_amd import _lib
+    from fem_taichi_amd.learned import E_REF
+    from fem_taichi_amd.runtime import stream_ptr
+    P = 5000
+    gen = torch.Generator().manual_seed(8)
+    r = (torch.randn(3 * P, generator=gen, dtype=torch.float64) * 1e3).cuda()
+    ef = torch.randn(P, generator=gen).cuda()
+    s = torch.sqrt(torch.mean(r * r)).clamp_min(1e-300)
+    x4 = torch.empty(P, 4, dtype=torch.bfloat16, device="cuda")
+    _lib.call("fem_meshconv_features_c4", r.data_ptr(), P, s.data_ptr(), ef.data_ptr(), x4.data_ptr(), stream_ptr())
+    ref = torch.empty(P, 4, dtype=torch.bfloat16, device="cuda")
+    ref[:, :3] = (r.view(P, 3) / s).to(torch.bfloat16)
+    ref[:, 3] = ef.to(torch.bfloat16)
+    assert torch.equal(x4, ref)
+    y = torch.randn(P, 16, generator=gen).cuda()
+    mask = (torch.rand(3 * P, generator=gen) > 0.1).double().cuda()
+    c = torch.empty(3 * P, dtype=torch.float64, device="cuda")
+    _lib.call("fem_meshconv_correction", y.data_ptr(), 16, P, s.data_ptr(), E_REF, mask.data_ptr(), c.data_ptr(),
+              stream_ptr())
+    cref = y[:, :3].reshape(-1).double() * (s / E_REF) * mask
+    assert torch.equal(c, cref)
