@@ -186,8 +186,9 @@ def test_static_mixed_fallback(setup):
 
 @pytest.mark.parametrize("nrhs", [1, 3, 8, 10, 16, 64])
 def test_spmm_multi(setup, nrhs):
-    """fem_bsr_spmm (two columns per thread for even counts from 8 on, thread
-    per (row, column) otherwise) and fem_mass_spmm against scipy."""
+    """fem_bsr_spmm (K staged in LDS for 16 columns, two columns per thread for
+    other even counts from 8 on, thread per (row, column) otherwise) and
+    fem_mass_spmm against scipy."""
     import torch
     from fem_taichi_amd import _lib
     from fem_taichi_amd.runtime import stream_ptr
@@ -201,6 +202,15 @@ def test_spmm_multi(setup, nrhs):
     _lib.call("fem_bsr_spmm", model.dev.ref, model.K_bsr("f64").data_ptr(), Xd.data_ptr(), Y.data_ptr(), nrhs,
               stream_ptr())
     assert rel(Y.cpu().numpy(), K @ X) < 1e-14
+    if nrhs == 16:
+        # the 16-column kernel (K staged in LDS) sums every column like the
+        # two-column kernel does for 8: bit-identical to the two halves
+        for h in range(2):
+            Xh = Xd[:, 8 * h:8 * h + 8].contiguous()
+            Yh = torch.empty_like(Xh)
+            _lib.call("fem_bsr_spmm", model.dev.ref, model.K_bsr("f64").data_ptr(), Xh.data_ptr(), Yh.data_ptr(), 8,
+                      stream_ptr())
+            assert torch.equal(Yh, Y[:, 8 * h:8 * h + 8])
     _lib.call("fem_mass_spmm", model.dev.ref, model.M_blocks().data_ptr(), Xd.data_ptr(), Y.data_ptr(), nrhs,
               stream_ptr())
     assert rel(Y.cpu().numpy(), M @ X) < 1e-14
