@@ -223,6 +223,80 @@ smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __r
   }
 }
 
+// The same passes for 16 columns (MODE 0 and 1): the 8 lanes of a row take
+// two columns each and stage the row's K blocks in LDS eight at a time with
+// coalesced loads, instead of every lane loading every block (16 lanes per
+// row read the same 36 bytes).  Same sums in the same order per column as
+// smooth_multi_kernel (bit-identical); on the 2.3M-node blade's finest level
+// 1.63 ms instead of 1.97 ms (at 8 columns no gain).
+constexpr int kSm16Team = 8, kSm16Rows = 64 / kSm16Team, kSm16Stride = 9 * kSm16Team + 1;
+template <int MODE>
+__global__ void __launch_bounds__(kThreads)
+smooth_multi16_kernel(fem_mesh m, const float* __restrict__ vals, const float* __restrict__ dinv, float w,
+                      const float* __restrict__ X, const float* __restrict__ B, float* __restrict__ OUT) {
+  constexpr int nb = 16;
+  __shared__ float sk[kThreads / 64][kSm16Rows][kSm16Stride];
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r = l / kSm16Team, j = l % kSm16Team;
+  const int64_t i = ((int64_t)blockIdx.x * kThreads + threadIdx.x) / kSm16Team;
+  if (i >= m.num_node) return;       // whole row teams leave together
+  const int64_t sm = m.sm_node[i];
+  const int co = m.co_node[i];
+  const int c = 2 * j;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f, b0 = 0.f, b1 = 0.f, b2 = 0.f;
+  float* s = &sk[wv][r][0];
+  for (int c0 = 0; c0 < co; c0 += kSm16Team) {
+    const int nblk = min(kSm16Team, co - c0);
+    const float* src = vals + 9 * (sm + c0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int d = j + kSm16Team * k;
+      if (d < 9 * nblk) s[d] = src[d];
+    }
+    const int colj = j < nblk ? m.re_node[sm + c0 + j] : 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 4
+    for (int b = 0; b < kSm16Team; ++b) {
+      if (b < nblk) {
+        const float* v = s + 9 * b;
+        const int64_t xc = 3 * (int64_t)__shfl(colj, b, kSm16Team) * nb + c;
+        const float2 x0 = *reinterpret_cast<const float2*>(X + xc);
+        const float2 x1 = *reinterpret_cast<const float2*>(X + xc + nb);
+        const float2 x2 = *reinterpret_cast<const float2*>(X + xc + 2 * nb);
+        a0 += v[0] * x0.x + v[1] * x1.x + v[2] * x2.x;
+        a1 += v[3] * x0.x + v[4] * x1.x + v[5] * x2.x;
+        a2 += v[6] * x0.x + v[7] * x1.x + v[8] * x2.x;
+        b0 += v[0] * x0.y + v[1] * x1.y + v[2] * x2.y;
+        b1 += v[3] * x0.y + v[4] * x1.y + v[5] * x2.y;
+        b2 += v[6] * x0.y + v[7] * x1.y + v[8] * x2.y;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  const float acc[2][3] = {{a0, a1, a2}, {b0, b1, b2}};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int64_t o = 3 * i * nb + c + h;
+    const float r0 = B[o] - acc[h][0], r1 = B[o + nb] - acc[h][1], r2 = B[o + 2 * nb] - acc[h][2];
+    if (MODE == 0) {
+      OUT[o] = r0;
+      OUT[o + nb] = r1;
+      OUT[o + 2 * nb] = r2;
+    } else {
+      const float* __restrict__ D = dinv + 9 * i;
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        const float d = w * (D[3 * p] * r0 + D[3 * p + 1] * r1 + D[3 * p + 2] * r2);
+        OUT[o + p * nb] = X[o + p * nb] + d;
+      }
+    }
+  }
+}
+
 // Y = A X for the coarsest level's dense inverse, X and Y n x nb row-major.
 // One wave per row of A: lane l takes k = l, l + 64, ... (A's row read
 // coalesced, X's rows as contiguous runs of CB columns), CB column sums per
@@ -354,10 +428,19 @@ extern "C" int fem_mg_smooth_multi(const fem_mesh* mesh, const float* vals, cons
   if (T == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
   const int nb = (int)nrhs;
-  // One column per thread: measured on the 2.3M-node blade against two and
-  // four columns per thread (residual mode, unrolled block loop), 8 columns
-  // take 1.31 ms and 16 columns 1.94 ms (2 per thread: 1.98 ms); the X
-  // gathers, not the K blocks, set the time.
+  // 16 columns, residual or update: K staged in LDS, two columns per lane.
+  // Otherwise one column per thread: measured on the 2.3M-node blade against
+  // two and four columns per thread (residual mode, unrolled block loop), 8
+  // columns take 1.31 ms.
+  if (nb == 16 && mode != 2 && reinterpret_cast<uintptr_t>(X) % 8 == 0) {
+    const int64_t T16 = mesh->num_node * kSm16Team;
+    if (mode == 0)
+      smooth_multi16_kernel<0><<<ceil_div(T16, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT);
+    else
+      smooth_multi16_kernel<1><<<ceil_div(T16, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT);
+    FEM_LAUNCH_CHECK();
+    return FEM_OK;
+  }
   if (mode == 0)
     smooth_multi_kernel<0><<<ceil_div(T, kThreads), kThreads, 0, st>>>(*mesh, vals, dinv, omega, X, B, OUT, nb);
   else if (mode == 1)

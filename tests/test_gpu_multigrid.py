@@ -118,6 +118,15 @@ def test_smoother_multi_matches_single(gpu, nb):
             a, r = OUT[:, c].cpu().numpy().astype(np.float64), o.cpu().numpy().astype(np.float64)
             fin = np.abs(r) < 1e15          # penalised residual rows: 1e20 x, summed in other orders
             assert rel(a[fin], r[fin]) < 1e-5, (mode, c)
+        if nb == 16:
+            # the 16-column kernel (K staged in LDS) sums each column like the
+            # one-column-per-thread kernel: bit-identical to two 8-column calls
+            for h in range(2):
+                Xh, Bh = X[:, 8 * h:8 * h + 8].contiguous(), B[:, 8 * h:8 * h + 8].contiguous()
+                Oh = torch.empty_like(Xh)
+                _lib.call("fem_mg_smooth_multi", model.dev.ref, k32.data_ptr(), dinv.data_ptr(), 0.6, Xh.data_ptr(),
+                          Bh.data_ptr(), Oh.data_ptr(), 8, mode, st)
+                assert torch.equal(Oh, OUT[:, 8 * h:8 * h + 8]), (mode, h)
 
 
 def test_block_fcg_matches_scipy(gpu):
