@@ -123,6 +123,8 @@ SIGNATURES = {
     "fem_gemm_tn_work_bytes": (_I64, [_I64, _I64]),
     "fem_gemm_tn_f64": (_I32, [_P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P, _I64, _P]),
     "fem_gemm_tn_f64_noalloc": (_I32, [_P, _I64, _P, _I64, _I64, _I64, _I64, _P, _P]),
+    "fem_gemm_nn_f64": (_I32, [_P, _I64, _P, _I64, _P, _I64, _I64, _I64, _I64, _D, _D, _P]),
+    "fem_chol_upper_inv_f64": (_I32, [_P, _I64, _P, _P, _P]),
     "fem_fcg_work_bytes": (_I64, [_I64]),
     "fem_fcg_direction": (_I32, [_I64, _I64, _P, _I32, _P, _P, _P, _I32, _P, _I64, _P]),
     "fem_fcg_update": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),

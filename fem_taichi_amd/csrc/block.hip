@@ -675,3 +675,131 @@ extern "C" int fem_fcg_update(int64_t n, int64_t nb, const double* P, const doub
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
+
+namespace fem {
+namespace {
+
+// ---- block Lanczos updates (SOLVE_eigen.py:55, 60-61, 66-71) ------------
+// Y = alpha A C + beta Y, A tall (n x k, row stride lda), C small (k x q).
+// A work-group owns 256 rows, one per thread.  The k columns go in chunks of
+// kNnChunk: the group stages its rows' chunk in LDS with coalesced loads
+// (16 consecutive threads read one row's 128 contiguous bytes) and the
+// chunk of C beside it; each thread then accumulates its row's q outputs
+// from LDS (C entries are broadcast reads).  Every output is a sum in
+// ascending column order, so results are deterministic.
+constexpr int kNnT = 256;
+constexpr int kNnChunk = 16;
+
+template <int QMAX>
+__global__ void __launch_bounds__(kNnT)
+gemm_nn_kernel(const double* __restrict__ A, int64_t lda, const double* __restrict__ C, int64_t ldc,
+               double* __restrict__ Y, int64_t ldy, int64_t n, int k, int q, double alpha, double beta) {
+  __shared__ double sA[kNnT * (kNnChunk + 1)];
+  __shared__ double sC[kNnChunk * QMAX];
+  const int64_t row0 = (int64_t)blockIdx.x * kNnT;
+  const int tid = threadIdx.x;
+  double acc[QMAX];
+#pragma unroll
+  for (int c = 0; c < QMAX; ++c) acc[c] = 0.0;
+  for (int j0 = 0; j0 < k; j0 += kNnChunk) {
+    const int kc = k - j0 < kNnChunk ? k - j0 : kNnChunk;
+    __syncthreads();
+#pragma unroll 4
+    for (int i = 0; i < kNnChunk; ++i) {
+      const int e = tid + i * kNnT;               // row e / 16, column e % 16 of the chunk
+      const int r = e / kNnChunk, j = e - r * kNnChunk;
+      const int64_t row = row0 + r;
+      sA[r * (kNnChunk + 1) + j] = (row < n && j < kc) ? A[row * lda + j0 + j] : 0.0;
+    }
+    for (int e = tid; e < kNnChunk * QMAX; e += kNnT) {
+      const int j = e / QMAX, c = e - j * QMAX;
+      sC[e] = (j < kc && c < q) ? C[(int64_t)(j0 + j) * ldc + c] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kc; ++j) {
+      const double a = sA[tid * (kNnChunk + 1) + j];
+#pragma unroll
+      for (int c = 0; c < QMAX; ++c) acc[c] = fma(a, sC[j * QMAX + c], acc[c]);
+    }
+  }
+  const int64_t row = row0 + tid;
+  if (row >= n) return;
+  double* y = Y + row * ldy;
+#pragma unroll
+  for (int c = 0; c < QMAX; ++c) {
+    if (c < q) y[c] = beta == 0.0 ? alpha * acc[c] : fma(alpha, acc[c], beta * y[c]);
+  }
+}
+
+// Upper Cholesky factor R (A = R^T R) of a small SPD b x b matrix and its
+// inverse, in one work-group (SOLVE_eigen.py:52, 75: scipy cholesky, then the
+// inverse the next block is scaled by).  Columns are spread over the threads;
+// a non-positive pivot makes R and Rinv NaN (the caller checks the result).
+constexpr int kCholT = 64;
+
+__global__ void __launch_bounds__(kCholT)
+chol_upper_inv_kernel(const double* __restrict__ A, int b, double* __restrict__ R, double* __restrict__ Rinv) {
+  __shared__ double sR[64 * 64];
+  __shared__ double sX[64 * 64];
+  const int c = threadIdx.x;
+  for (int e = c; e < b * b; e += kCholT) sR[(e / b) * 64 + e % b] = A[e];
+  __syncthreads();
+  for (int j = 0; j < b; ++j) {
+    // row j of R: pivot first, then the rest of the row from the updated pivot
+    if (c == 0) {
+      const double d = sR[j * 64 + j];
+      sR[j * 64 + j] = d > 0.0 ? sqrt(d) : __longlong_as_double(0x7ff8000000000000LL);
+    }
+    __syncthreads();
+    const double piv = sR[j * 64 + j];
+    if (c > j && c < b) sR[j * 64 + c] /= piv;
+    __syncthreads();
+    // trailing update A_cc' -= R_jc R_jc' for c, c' > j (thread c owns column c)
+    if (c > j && c < b) {
+      const double rjc = sR[j * 64 + c];
+      for (int i = j + 1; i <= c; ++i) sR[i * 64 + c] -= sR[j * 64 + i] * rjc;
+    }
+    __syncthreads();
+  }
+  if (c < b) {
+    for (int i = 0; i < b; ++i) R[i * b + c] = i <= c ? sR[i * 64 + c] : 0.0;
+    // column c of R^-1 by back substitution, R x = e_c (x in column c of sX)
+    for (int i = b - 1; i >= 0; --i) {
+      double s = i == c ? 1.0 : 0.0;
+      for (int j = i + 1; j <= c; ++j) s -= sR[i * 64 + j] * sX[j * 64 + c];
+      sX[i * 64 + c] = i <= c ? s / sR[i * 64 + i] : 0.0;
+    }
+    for (int i = 0; i < b; ++i) Rinv[i * b + c] = sX[i * 64 + c];
+  }
+}
+
+}  // namespace
+}  // namespace fem
+
+extern "C" int fem_gemm_nn_f64(const double* A, int64_t lda, const double* C, int64_t ldc, double* Y, int64_t ldy,
+                               int64_t n, int64_t k, int64_t q, double alpha, double beta, void* stream) {
+  FEM_REQUIRE(n >= 0 && k >= 0 && q >= 1 && q <= 64 && k <= (1 << 20), FEM_E_INVALID,
+              "need n, k >= 0 and 1 <= q <= 64 (got n %lld, k %lld, q %lld)", (long long)n, (long long)k,
+              (long long)q);
+  FEM_REQUIRE(lda >= k && ldc >= q && ldy >= q, FEM_E_INVALID, "row strides lda %lld, ldc %lld, ldy %lld too small",
+              (long long)lda, (long long)ldc, (long long)ldy);
+  if (n == 0) return FEM_OK;
+  FEM_REQUIRE(Y && (k == 0 || (A && C)), FEM_E_INVALID, "null pointer");
+  hipStream_t st = as_stream(stream);
+  const unsigned grid = (unsigned)((n + kNnT - 1) / kNnT);
+  const int ki = (int)k, qi = (int)q;
+  if (q <= 8) gemm_nn_kernel<8><<<grid, kNnT, 0, st>>>(A, lda, C, ldc, Y, ldy, n, ki, qi, alpha, beta);
+  else if (q <= 16) gemm_nn_kernel<16><<<grid, kNnT, 0, st>>>(A, lda, C, ldc, Y, ldy, n, ki, qi, alpha, beta);
+  else if (q <= 32) gemm_nn_kernel<32><<<grid, kNnT, 0, st>>>(A, lda, C, ldc, Y, ldy, n, ki, qi, alpha, beta);
+  else gemm_nn_kernel<64><<<grid, kNnT, 0, st>>>(A, lda, C, ldc, Y, ldy, n, ki, qi, alpha, beta);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_chol_upper_inv_f64(const double* A, int64_t b, double* R, double* Rinv, void* stream) {
+  FEM_REQUIRE(b >= 1 && b <= 64, FEM_E_INVALID, "b must be in [1, 64] (got %lld)", (long long)b);
+  FEM_REQUIRE(A && R && Rinv, FEM_E_INVALID, "null pointer");
+  chol_upper_inv_kernel<<<1, kCholT, 0, as_stream(stream)>>>(A, (int)b, R, Rinv);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}

@@ -210,9 +210,30 @@ def gram(X, Y):
     return C
 
 
-def _chol_upper(A):
-    """scipy.linalg.cholesky(A) (upper, A = U^T U)."""
-    return torch.linalg.cholesky(A).transpose(0, 1).contiguous()
+def gemm_nn(A, C, Y=None, alpha=1.0, beta=0.0):
+    """Y = alpha A C + beta Y (fem_gemm_nn_f64): A (n, k) with unit column
+    stride (a column slice of Q is fine), C (k, q) small; a new Y when None."""
+    n, k = A.shape
+    q = C.shape[1]
+    C = C.contiguous()
+    if Y is None:
+        Y = torch.empty(n, q, dtype=torch.float64, device=A.device)
+        beta = 0.0
+    if A.stride(1) != 1 or Y.stride(1) != 1 or A.dtype != torch.float64 or Y.dtype != torch.float64:
+        raise ValueError("gemm_nn needs FP64 operands with unit column stride")
+    _lib.call("fem_gemm_nn_f64", A.data_ptr(), A.stride(0), C.data_ptr(), C.stride(0), Y.data_ptr(), Y.stride(0),
+              n, k, q, float(alpha), float(beta), stream_ptr())
+    return Y
+
+
+def chol_upper_inv(A):
+    """(R, R^-1) with A = R^T R, R upper (scipy.linalg.cholesky(A), SOLVE_eigen.py:52, 75), on the
+    device by fem_chol_upper_inv_f64; a matrix that is not positive definite gives NaN."""
+    A = A.contiguous()
+    R = torch.empty_like(A)
+    Rinv = torch.empty_like(A)
+    _lib.call("fem_chol_upper_inv_f64", A.data_ptr(), A.shape[0], R.data_ptr(), Rinv.data_ptr(), stream_ptr())
+    return R, Rinv
 
 
 def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
@@ -239,35 +260,39 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
         u = u0.to(device=d, dtype=torch.float64).contiguous()
     else:
         u = torch.as_tensor(np.asarray(u0, dtype=np.float64)).to(d)
-    v = _chol_upper(gram(u, ops.m_apply(u)))
-    u = u @ torch.linalg.inv(v)
+    _, vinv = chol_upper_inv(gram(u, ops.m_apply(u)))
+    u = gemm_nn(u, vinv)
     Q[:, :b] = u
     steps = int(m / b)
     for i in range(steps):
         v = ops.solve(ops.m_apply(u), inner_tol)
         Tii = gram(u, ops.m_apply(v))
         T[i * b:(i + 1) * b, i * b:(i + 1) * b] = Tii
-        v = torch.addmm(v, u, Tii, alpha=-1.0)
+        gemm_nn(u, Tii, v, -1.0, 1.0)                         # v -= u T_ii
         if i >= steps - 1:
             break
         k = i * b + b
         Qk = Q[:, :k]
-        v = torch.addmm(v, Qk, gram(Qk, ops.m_apply(v)), alpha=-1.0)
-        v = torch.addmm(v, Qk, gram(Qk, ops.m_apply(v)), alpha=-1.0)
-        t = _chol_upper(gram(v, ops.m_apply(v)))
+        gemm_nn(Qk, gram(Qk, ops.m_apply(v)), v, -1.0, 1.0)  # two passes of v -= Q (Q^T M v)
+        gemm_nn(Qk, gram(Qk, ops.m_apply(v)), v, -1.0, 1.0)
+        t, tinv = chol_upper_inv(gram(v, ops.m_apply(v)))
         T[(i + 1) * b:(i + 2) * b, i * b:(i + 1) * b] = t
         T[i * b:(i + 1) * b, (i + 1) * b:(i + 2) * b] = t.T
-        u = v @ torch.linalg.inv(t)
+        u = gemm_nn(v, tinv)
         Q[:, (i + 1) * b:(i + 2) * b] = u
     # T's entries are ~1/lambda (1e-11 .. 1e-13 for steel in mm/t/s units);
     # the device eigensolver loses relative accuracy at that scale, so it runs
     # on T / max|T| and theta is scaled back.
+    if not bool(torch.isfinite(T).all()):
+        raise np.linalg.LinAlgError("block Lanczos: a block Gram matrix u^T M u is not positive definite")
     scale = T.abs().max()
     scale = torch.where(scale > 0, scale, torch.ones_like(scale))
     theta, V = torch.linalg.eigh(T / scale)
     theta = theta * scale
     lam = 1.0 / theta
-    X = Q @ V
+    X = torch.empty(n, m, dtype=torch.float64, device=d)
+    for c0 in range(0, m, 64):                                 # Ritz vectors Q V
+        gemm_nn(Q, V[:, c0:c0 + 64], X[:, c0:c0 + 64])
     ind = torch.argsort(lam)
     return lam[ind][:num_eigen], X[:, ind][:, :num_eigen], T
 

@@ -342,6 +342,23 @@ int fem_gemm_tn_f64(const double* A, int64_t lda, const double* B, int64_t ldb, 
 int fem_gemm_tn_f64_noalloc(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p,
                             int64_t q, double* C, void* stream);
 
+/* Y = alpha * A C + beta * Y for a tall A (n x k, row stride lda) and a small
+ * C (k x q, row stride ldc), FP64, Y (n x q, row stride ldy) not overlapping
+ * A: the block updates of block Lanczos (SOLVE_eigen.py:55 u <- u R^-1, :60
+ * v -= u T_ii, :66-71 the two reorthogonalisation passes v -= Q (Q^T M v)).
+ * Each output is summed in ascending column order (deterministic).
+ * q <= 64; beta == 0 does not read Y. */
+int fem_gemm_nn_f64(const double* A, int64_t lda, const double* C, int64_t ldc, double* Y, int64_t ldy,
+                    int64_t n, int64_t k, int64_t q, double alpha, double beta, void* stream);
+
+/* Upper Cholesky factor R (A = R^T R, R upper triangular, zeros below) of a
+ * small SPD b x b matrix A (row-major, device) and its inverse Rinv, both b x b
+ * row-major: the scipy.linalg.cholesky and the inverse that scale each new
+ * Lanczos block (SOLVE_eigen.py:52-53, 75-77).  b <= 64, one work-group.  A
+ * non-positive pivot makes R and Rinv NaN from that row on (no host sync;
+ * the caller checks the result). */
+int fem_chol_upper_inv_f64(const double* A, int64_t b, double* R, double* Rinv, void* stream);
+
 /* Flexible CG, FCG(1), vector steps on nb column-interleaved vectors (n rows,
  * entry (i, c) at i*nb + c), around an outside preconditioner (the multigrid
  * V-cycle).  The reference's static and shift-invert solves are direct

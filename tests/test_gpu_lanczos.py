@@ -181,3 +181,45 @@ def test_block_solve_wider_than_kernels(gpu, inner):
     assert X.shape == (n, 70) and len(ops.iterations) == 1
     res = np.linalg.norm(B - K @ X, axis=0) / np.linalg.norm(B, axis=0)
     assert res.max() < 1e-8, res.max()
+
+
+@pytest.mark.parametrize("n,k,q", [(1, 1, 1), (1000, 37, 10), (70001, 200, 16), (5000, 16, 64), (3, 300, 8)])
+def test_gemm_nn_matches_numpy(gpu, n, k, q):
+    """fem_gemm_nn_f64 (the Lanczos block updates) against NumPy, including a
+    strided A (a column slice of a wider Q), beta = 1 and alpha = -1."""
+    import torch
+    from fem_taichi_amd.solve_eigen import gemm_nn
+    rng = np.random.default_rng(n + k + q)
+    Qw = rng.standard_normal((n, k + 5))
+    C = rng.standard_normal((k, q))
+    Y0 = rng.standard_normal((n, q))
+    Qd = torch.as_tensor(Qw).cuda()
+    A = Qd[:, :k]                          # row stride k + 5
+    Y = gemm_nn(A, torch.as_tensor(C).cuda())
+    ref = Qw[:, :k] @ C
+    assert np.abs(Y.cpu().numpy() - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max()) * k
+    Yd = torch.as_tensor(Y0).cuda()
+    gemm_nn(A, torch.as_tensor(C).cuda(), Yd, -1.0, 1.0)
+    ref2 = Y0 - ref
+    assert np.abs(Yd.cpu().numpy() - ref2).max() <= 1e-13 * max(1.0, np.abs(ref2).max()) * k
+
+
+@pytest.mark.parametrize("b", [1, 4, 10, 16, 64])
+def test_chol_upper_inv_matches_scipy(gpu, b):
+    import scipy.linalg as sl
+    import torch
+    from fem_taichi_amd.solve_eigen import chol_upper_inv
+    rng = np.random.default_rng(b)
+    G = rng.standard_normal((b + 3, b))
+    A = G.T @ G + b * np.eye(b) * 1e-3
+    R, Rinv = chol_upper_inv(torch.as_tensor(A).cuda())
+    R, Rinv = R.cpu().numpy(), Rinv.cpu().numpy()
+    Rref = sl.cholesky(A)                  # upper, A = R^T R (the reference's call)
+    assert np.allclose(np.tril(R, -1), 0.0, atol=0.0)
+    assert np.abs(R - Rref).max() <= 1e-12 * np.abs(Rref).max()
+    assert np.abs(Rinv @ R - np.eye(b)).max() <= 1e-12 * b
+    # not positive definite: NaN, no exception inside the kernel
+    Bad = A.copy()
+    Bad[b - 1, b - 1] = -1.0
+    Rb, _ = chol_upper_inv(torch.as_tensor(Bad).cuda())
+    assert np.isnan(Rb.cpu().numpy()[b - 1, b - 1])
