@@ -56,3 +56,20 @@ def test_grid_transfer_struct_matches_header():
     body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     fields = re.findall(r"(\w+)\s*(?:\[\d+\])?\s*;", body)
     assert fields == [f[0] for f in _lib.GridTransfer._fields_]
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libfem355.so not built")
+def test_assembly_rejects_patterns_beyond_int32_values():
+    """The assembly's CSR/BSR value index 9*s is int32: a pattern of 4x the
+    500x60x75 blade (61.4M node-pair blocks there, 9*nnzb = 552.6M) is refused
+    with FEM_E_LIMIT before any device work (argument check only)."""
+    from fem_taichi_amd import _lib
+    m = _lib.FemMesh()
+    m.num_node, m.num_elem = 4 * 2322636, 4 * 2250000
+    m.nnzb = 4 * 61399906
+    rc = _lib.lib().fem_hex8_assemble(ctypes.byref(m), None, 2e5, 0.3, 7.8e-9, ctypes.c_void_p(64), 0, 0, None, None)
+    assert rc == _lib.FEM_E_LIMIT
+    assert "int32" in _lib.last_error()
+    m.nnzb = 61399906                      # the blade itself passes that check (then fails on the NULL pattern)
+    rc = _lib.lib().fem_hex8_assemble(ctypes.byref(m), None, 2e5, 0.3, 7.8e-9, ctypes.c_void_p(64), 0, 0, None, None)
+    assert rc == _lib.FEM_E_INVALID
