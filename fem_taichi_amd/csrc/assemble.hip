@@ -77,6 +77,19 @@ __device__ __forceinline__ void dn_factored(int k, const double fp[3], const dou
   dn[2] = 0.125 * node_sign(k, 2) * (f0 * f1);
 }
 
+// 8 dN_k/dxi: the same without the 1/8.  With it J' = 8 J, whose cofactor
+// inverse is J^-1 / 8 and whose determinant is 512 det J, all exactly (powers
+// of two), so dN8 J'^-1 = dN J^-1 bit for bit and no scaling instruction runs.
+__device__ __forceinline__ void dn8_factored(int k, const double fp[3], const double fm[3], double dn[3]) {
+  const double f0 = node_sign(k, 0) > 0 ? fp[0] : fm[0];
+  const double f1 = node_sign(k, 1) > 0 ? fp[1] : fm[1];
+  const double f2 = node_sign(k, 2) > 0 ? fp[2] : fm[2];
+  dn[0] = node_sign(k, 0) * (f1 * f2);
+  dn[1] = node_sign(k, 1) * (f0 * f2);
+  dn[2] = node_sign(k, 2) * (f0 * f1);
+}
+constexpr double kInv512 = 1.0 / 512.0;
+
 template <typename T>
 __device__ __forceinline__ void gram(const T* __restrict__ s_a, const T* __restrict__ s_w, int c, T G[9]) {
   // G += sum_g w_g a_gj (x) a_gk for contribution c (scaled by 0 when absent)
@@ -286,11 +299,11 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
 #pragma unroll
     for (int q = 0; q < 9; ++q) J[q] = 0.0;
 #pragma unroll
-    for (int k = 1; k < 8; ++k) {   // J = sum_k (x_k - x_0) (x) dN_k (the x_0 term sums to zero)
+    for (int k = 1; k < 8; ++k) {   // 8 J = sum_k (x_k - x_0) (x) 8 dN_k (the x_0 term sums to zero)
       const int64_t nk = nodes[k];
       const double xr[3] = {m.coords[nk * 3] - x00, m.coords[nk * 3 + 1] - x01, m.coords[nk * 3 + 2] - x02};
       double dn[3];
-      dn_factored(k, fp, fm, dn);
+      dn8_factored(k, fp, fm, dn);
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -298,7 +311,7 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
     }
     if (WANT_K) {
       double Ji[9];
-      const double det = inv3(J, Ji);
+      const double det = inv3(J, Ji) * kInv512;     // det J
       s_w[le * 8 + g] = (T)(Ee * det);
       if (WANT_M) s_wm[le * 8 + g] = rho * det;
       // recompute the factors rather than keep 24 derivatives live across the inverse
@@ -307,13 +320,13 @@ hex8_assemble_kernel(fem_mesh m, const double* __restrict__ young, double young_
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         double dn[3];
-        dn_factored(k, fp, fm, dn);
+        dn8_factored(k, fp, fm, dn);
         T* dst = s_a + (le * 8 + k) * a_stride<T>() + 3 * g;
 #pragma unroll
         for (int a = 0; a < 3; ++a) dst[a] = (T)(dn[0] * Ji[0 * 3 + a] + dn[1] * Ji[1 * 3 + a] + dn[2] * Ji[2 * 3 + a]);
       }
     } else if (WANT_M) {
-      s_wm[le * 8 + g] = rho * det3(J);
+      s_wm[le * 8 + g] = rho * (det3(J) * kInv512);
     }
   }
   __syncthreads();

@@ -297,41 +297,41 @@ smooth_multi16_kernel(fem_mesh m, const float* __restrict__ vals, const float* _
   }
 }
 
-// Y = A X for the coarsest level's dense inverse, X and Y n x nb row-major.
-// One wave per row of A: lane l takes k = l, l + 64, ... (A's row read
-// coalesced, X's rows as contiguous runs of CB columns), CB column sums per
-// lane, then a fixed butterfly over the wave; columns in chunks of CB.
-constexpr int kGemmCB = 8;
+// Y = A X, A dense n x n row-major (the coarsest level's inverse), X and Y
+// n x nb with the columns fastest.  A work-group computes a 16-row x
+// 16-column tile of Y; A's rows and X's rows go through LDS in chunks of 64
+// (coalesced: 64 threads read one 256-byte row piece of A), and thread (r, c)
+// sums its output over the chunk in ascending k (deterministic).
+constexpr int kGemmRows = 16, kGemmCols = 16, kGemmK = 64;
 __global__ void __launch_bounds__(kThreads)
 gemm_nb_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__ X, float* __restrict__ Y, int nb) {
-  const int64_t row = (int64_t)blockIdx.x * (kThreads / 64) + threadIdx.x / 64;
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;                       // whole waves leave together
-  const float* __restrict__ a = A + row * n;
-  for (int c0 = 0; c0 < nb; c0 += kGemmCB) {
-    const int cw = nb - c0 < kGemmCB ? nb - c0 : kGemmCB;
-    float s[kGemmCB];
+  __shared__ float sA[kGemmRows * (kGemmK + 1)];
+  __shared__ float sX[kGemmK * kGemmCols];
+  const int tid = threadIdx.x;
+  const int r = tid / kGemmCols, c = tid % kGemmCols;
+  const int64_t row0 = (int64_t)blockIdx.x * kGemmRows;
+  const int col0 = blockIdx.y * kGemmCols;
+  float s = 0.f;
+  for (int64_t k0 = 0; k0 < n; k0 += kGemmK) {
+    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kGemmCB; ++q) s[q] = 0.f;
-    for (int64_t k = lane; k < n; k += 64) {
-      const float ak = a[k];
-      const float* __restrict__ xk = X + k * nb + c0;
-#pragma unroll
-      for (int q = 0; q < kGemmCB; ++q)
-        if (q < cw) s[q] += ak * xk[q];
+    for (int i = 0; i < kGemmRows * kGemmK / kThreads; ++i) {
+      const int e = tid + i * kThreads, ra = e / kGemmK, ka = e % kGemmK;
+      const int64_t gr = row0 + ra, gk = k0 + ka;
+      sA[ra * (kGemmK + 1) + ka] = (gr < n && gk < n) ? A[gr * n + gk] : 0.f;
     }
 #pragma unroll
-    for (int q = 0; q < kGemmCB; ++q)
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_xor(s[q], o, 64);
-    if (lane < cw) {
-      float v = s[0];
-#pragma unroll
-      for (int q = 1; q < kGemmCB; ++q)
-        if (lane == q) v = s[q];
-      Y[row * nb + c0 + lane] = v;
+    for (int i = 0; i < kGemmK * kGemmCols / kThreads; ++i) {
+      const int e = tid + i * kThreads, kx = e / kGemmCols, cx = e % kGemmCols;
+      const int64_t gk = k0 + kx;
+      sX[e] = (gk < n && col0 + cx < nb) ? X[gk * nb + col0 + cx] : 0.f;
     }
+    __syncthreads();
+#pragma unroll 16
+    for (int k = 0; k < kGemmK; ++k) s = fmaf(sA[r * (kGemmK + 1) + k], sX[k * kGemmCols + c], s);
   }
+  const int64_t row = row0 + r;
+  if (row < n && col0 + c < nb) Y[row * nb + col0 + c] = s;
 }
 
 // y = A x, A dense n x n row-major; one wave per row.
@@ -457,7 +457,8 @@ extern "C" int fem_dense_gemm_nb_f32(const float* A, int64_t n, const float* X, 
   FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   FEM_REQUIRE(X != Y, FEM_E_INVALID, "Y must not alias X");
   if (n == 0) return FEM_OK;
-  gemm_nb_kernel<<<ceil_div(n, kThreads / 64), kThreads, 0, as_stream(stream)>>>(A, n, X, Y, (int)nrhs);
+  const dim3 grid((unsigned)ceil_div(n, kGemmRows), (unsigned)ceil_div(nrhs, kGemmCols));
+  gemm_nb_kernel<<<grid, kThreads, 0, as_stream(stream)>>>(A, n, X, Y, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
