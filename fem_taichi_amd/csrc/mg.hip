@@ -97,14 +97,15 @@ __device__ __forceinline__ float axis_weight(const int32_t* __restrict__ left, c
 }
 
 // coarse[I] = sum over fine nodes f of P[f, I] fine[f] (thread per coarse node).
+// Launched on a (ceil(nc_x nb / 256), nc_y, nc_z) grid: the node's y and z
+// come from the block indices, x and the column from one 32-bit division.
 __global__ void __launch_bounds__(kThreads)
 restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __restrict__ coarse, int nb) {
-  const int64_t tI = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  const int64_t ncx = t.nc[0], ncy = t.nc[1];
-  if (tI >= ncx * ncy * t.nc[2] * nb) return;
-  const int64_t I = tI / nb;
-  const int col = (int)(tI - I * nb);
-  const int cx = (int)(I % ncx), cy = (int)((I / ncx) % ncy), cz = (int)(I / (ncx * ncy));
+  const uint32_t tx = blockIdx.x * kThreads + threadIdx.x;
+  if (tx >= (uint32_t)t.nc[0] * (uint32_t)nb) return;
+  const int cx = (int)(tx / (uint32_t)nb), col = (int)(tx - (uint32_t)cx * nb);
+  const int cy = blockIdx.y, cz = blockIdx.z;
+  const int64_t I = cx + (int64_t)t.nc[0] * (cy + (int64_t)t.nc[1] * cz);
   const int32_t* lx = t.left;
   const int32_t* ly = lx + t.nf[0];
   const int32_t* lz = ly + t.nf[1];
@@ -138,12 +139,11 @@ restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __re
 // fine[f] += sum over the (at most 8) coarse nodes I of P[f, I] coarse[I].
 __global__ void __launch_bounds__(kThreads)
 prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float* __restrict__ fine, int nb) {
-  const int64_t tf = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-  const int64_t nfx = t.nf[0], nfy = t.nf[1];
-  if (tf >= nfx * nfy * t.nf[2] * nb) return;
-  const int64_t f = tf / nb;
-  const int col = (int)(tf - f * nb);
-  const int fx = (int)(f % nfx), fy = (int)((f / nfx) % nfy), fz = (int)(f / (nfx * nfy));
+  const uint32_t tx = blockIdx.x * kThreads + threadIdx.x;     // grid (ceil(nf_x nb / 256), nf_y, nf_z)
+  if (tx >= (uint32_t)t.nf[0] * (uint32_t)nb) return;
+  const int fx = (int)(tx / (uint32_t)nb), col = (int)(tx - (uint32_t)fx * nb);
+  const int fy = blockIdx.y, fz = blockIdx.z;
+  const int64_t f = fx + (int64_t)t.nf[0] * (fy + (int64_t)t.nf[1] * fz);
   const int32_t* lx = t.left;
   const int32_t* ly = lx + t.nf[0];
   const int32_t* lz = ly + t.nf[1];
@@ -188,7 +188,8 @@ smooth_multi_kernel(fem_mesh m, const float* __restrict__ vals, const float* __r
                     const float* __restrict__ X, const float* __restrict__ B, float* __restrict__ OUT, int nb) {
   const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
   if (t >= m.num_node * nb) return;
-  const int64_t i = t / nb;
+  // (node, column) of the thread; a 32-bit division whenever the count fits
+  const int64_t i = t < ((int64_t)1 << 32) ? (int64_t)((uint32_t)t / (uint32_t)nb) : t / nb;
   const int c = (int)(t - i * nb);
   const int64_t o = 3 * i * nb + c;
   float r0 = B[o], r1 = B[o + nb], r2 = B[o + 2 * nb];
@@ -299,10 +300,13 @@ smooth_multi16_kernel(fem_mesh m, const float* __restrict__ vals, const float* _
 
 // Y = A X, A dense n x n row-major (the coarsest level's inverse), X and Y
 // n x nb with the columns fastest.  A work-group computes a 16-row x
-// 16-column tile of Y; A's rows and X's rows go through LDS in chunks of 64
-// (coalesced: 64 threads read one 256-byte row piece of A), and thread (r, c)
-// sums its output over the chunk in ascending k (deterministic).
-constexpr int kGemmRows = 16, kGemmCols = 16, kGemmK = 64;
+// 16-column tile of Y over k in chunks of 256: each chunk's A rows (16 x 256,
+// 64 threads per row, coalesced) and X rows (256 x 16) go to LDS, and the
+// next chunk is already loaded into registers while this one is summed, so
+// the work-group's chain of dependent loads is n / 256 long.  Thread (r, c)
+// sums its output in ascending k (deterministic).
+constexpr int kGemmRows = 16, kGemmCols = 16, kGemmK = 256;
+constexpr int kGemmA = kGemmRows * kGemmK / kThreads, kGemmX = kGemmK * kGemmCols / kThreads;   // 16, 16
 __global__ void __launch_bounds__(kThreads)
 gemm_nb_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__ X, float* __restrict__ Y, int nb) {
   __shared__ float sA[kGemmRows * (kGemmK + 1)];
@@ -311,22 +315,34 @@ gemm_nb_kernel(const float* __restrict__ A, int64_t n, const float* __restrict__
   const int r = tid / kGemmCols, c = tid % kGemmCols;
   const int64_t row0 = (int64_t)blockIdx.x * kGemmRows;
   const int col0 = blockIdx.y * kGemmCols;
+  float ra[kGemmA], rx[kGemmX];
+  auto load = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < kGemmA; ++i) {
+      const int e = tid + i * kThreads, ar = e / kGemmK, ak = e % kGemmK;
+      const int64_t gr = row0 + ar, gk = k0 + ak;
+      ra[i] = (gr < n && gk < n) ? A[gr * n + gk] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < kGemmX; ++i) {
+      const int e = tid + i * kThreads, xk = e / kGemmCols, xc = e % kGemmCols;
+      const int64_t gk = k0 + xk;
+      rx[i] = (gk < n && col0 + xc < nb) ? X[gk * nb + col0 + xc] : 0.f;
+    }
+  };
   float s = 0.f;
+  load(0);
   for (int64_t k0 = 0; k0 < n; k0 += kGemmK) {
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < kGemmRows * kGemmK / kThreads; ++i) {
-      const int e = tid + i * kThreads, ra = e / kGemmK, ka = e % kGemmK;
-      const int64_t gr = row0 + ra, gk = k0 + ka;
-      sA[ra * (kGemmK + 1) + ka] = (gr < n && gk < n) ? A[gr * n + gk] : 0.f;
+    for (int i = 0; i < kGemmA; ++i) {
+      const int e = tid + i * kThreads;
+      sA[(e / kGemmK) * (kGemmK + 1) + e % kGemmK] = ra[i];
     }
 #pragma unroll
-    for (int i = 0; i < kGemmK * kGemmCols / kThreads; ++i) {
-      const int e = tid + i * kThreads, kx = e / kGemmCols, cx = e % kGemmCols;
-      const int64_t gk = k0 + kx;
-      sX[e] = (gk < n && col0 + cx < nb) ? X[gk * nb + col0 + cx] : 0.f;
-    }
+    for (int i = 0; i < kGemmX; ++i) sX[tid + i * kThreads] = rx[i];
     __syncthreads();
+    if (k0 + kGemmK < n) load(k0 + kGemmK);
 #pragma unroll 16
     for (int k = 0; k < kGemmK; ++k) s = fmaf(sA[r * (kGemmK + 1) + k], sX[k * kGemmCols + c], s);
   }
@@ -398,7 +414,10 @@ extern "C" int fem_mg_restrict(const fem_grid_transfer* t, const float* fine, fl
   FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
   FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   const int64_t Nc = (int64_t)t->nc[0] * t->nc[1] * t->nc[2];
-  restrict_kernel<<<ceil_div(Nc * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(*t, fine, coarse, (int)nrhs);
+  if (Nc == 0) return FEM_OK;
+  FEM_REQUIRE(t->nc[1] <= 65535 && t->nc[2] <= 65535, FEM_E_LIMIT, "coarse grid beyond 65535 nodes along y or z");
+  const dim3 grid((unsigned)ceil_div((int64_t)t->nc[0] * nrhs, kThreads), (unsigned)t->nc[1], (unsigned)t->nc[2]);
+  restrict_kernel<<<grid, kThreads, 0, as_stream(stream)>>>(*t, fine, coarse, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -410,7 +429,10 @@ extern "C" int fem_mg_prolong_add(const fem_grid_transfer* t, const float* coars
   FEM_REQUIRE(fine && coarse, FEM_E_INVALID, "null vector");
   FEM_REQUIRE(nrhs >= 1 && nrhs <= 64, FEM_E_INVALID, "nrhs must be in [1, 64]");
   const int64_t Nf = (int64_t)t->nf[0] * t->nf[1] * t->nf[2];
-  prolong_add_kernel<<<ceil_div(Nf * nrhs, kThreads), kThreads, 0, as_stream(stream)>>>(*t, coarse, fine, (int)nrhs);
+  if (Nf == 0) return FEM_OK;
+  FEM_REQUIRE(t->nf[1] <= 65535 && t->nf[2] <= 65535, FEM_E_LIMIT, "fine grid beyond 65535 nodes along y or z");
+  const dim3 grid((unsigned)ceil_div((int64_t)t->nf[0] * nrhs, kThreads), (unsigned)t->nf[1], (unsigned)t->nf[2]);
+  prolong_add_kernel<<<grid, kThreads, 0, as_stream(stream)>>>(*t, coarse, fine, (int)nrhs);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

@@ -78,6 +78,20 @@ def grid_coords(nx, ny, nz, h=1.0):
     return xyz * h
 
 
+def is_grid_mesh(elem_idx, num_node, dims) -> bool:
+    """True when (E, 8) node indices over ``num_node`` nodes are exactly the
+    grid of ``dims`` = (nx, ny, nz) elements in the numbering of
+    grid_connectivity (node ids x-fastest, elements x-fastest, the I..P
+    corner order)."""
+    nx, ny, nz = (int(d) for d in dims)
+    if min(nx, ny, nz) < 1 or num_node != (nx + 1) * (ny + 1) * (nz + 1):
+        return False
+    elem_idx = np.asarray(elem_idx)
+    if elem_idx.shape != (nx * ny * nz, 8):
+        return False
+    return bool(np.array_equal(elem_idx, grid_connectivity(nx, ny, nz)))
+
+
 def structured_block(nx, ny, nz, h=1.0, jitter=0.0, seed=0, E=200000.0, E_sigma=0.0,
                      nu=0.3, rho=7.8e-9, fix_face='z0', load_face='zmax', total_load=1000.0,
                      twist_deg=0.0, omegas=(0.0, 0.0, 0.0), name='grid') -> StructuredGrid:

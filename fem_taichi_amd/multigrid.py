@@ -335,8 +335,7 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_smooth_multi", dev.ref, None, dv, w, None, b[l].data_ptr(), x[l].data_ptr(), nb, 2, st)
-            _lib.call("fem_mg_smooth_multi", dev.ref, k, dv, w, x[l].data_ptr(), b[l].data_ptr(), t[l].data_ptr(), nb,
-                      0, st)
+            _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 0, st)
             _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, t[l].data_ptr(), b[l + 1].data_ptr(), nb, st)
         _lib.call("fem_dense_gemm_nb_f32", self.coarse_inv.data_ptr(), b[L - 1].shape[0], b[L - 1].data_ptr(),
                   x[L - 1].data_ptr(), nb, st)
@@ -344,10 +343,21 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, x[l + 1].data_ptr(), x[l].data_ptr(), nb, st)
-            _lib.call("fem_mg_smooth_multi", dev.ref, k, dv, w, x[l].data_ptr(), b[l].data_ptr(), t[l].data_ptr(), nb,
-                      1, st)
+            _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
             x[l], t[l] = t[l], x[l]
         return x[0]
+
+
+def _smooth_multi(dev, k, dv, w, x, b, out, nb, mode, st):
+    """A residual (mode 0) or update (mode 1) pass of the multi-column V-cycle:
+    the structured-grid 16-column kernel (fem_mg_smooth16_grid) on node grids
+    of that numbering, else fem_mg_smooth_multi."""
+    grid = getattr(dev, "grid_nodes", None)
+    if nb == 16 and grid is not None and x.data_ptr() % 16 == 0:
+        _lib.call("fem_mg_smooth16_grid", dev.ref, grid[0], grid[1], grid[2], k, dv, w, x.data_ptr(), b.data_ptr(),
+                  out.data_ptr(), mode, st)
+    else:
+        _lib.call("fem_mg_smooth_multi", dev.ref, k, dv, w, x.data_ptr(), b.data_ptr(), out.data_ptr(), nb, mode, st)
 
 
 class FcgSteps:
@@ -508,8 +518,8 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     so it runs to ``maxiter`` and is neither converged nor stagnated (the
     caller's fallback finishes it).  Returns (X, info) with per-column
     ``converged``, ``stagnated`` and ``rel_residual``."""
+    from . import solvers
     from .learned import free_dof_mask
-    st = stream_ptr()
     n, nb = B.shape
     if free is None:
         free = free_dof_mask(model)
@@ -522,8 +532,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     def spmm(X):
         X = X.contiguous()
         Y = torch.empty_like(X)
-        _lib.call("fem_bsr_spmm", K64.mesh.ref, K64.vals.data_ptr(), X.data_ptr(), Y.data_ptr(), nb, st)
-        return Y
+        return solvers.bsr_spmm(K64.mesh, K64.vals, X, Y, nb)
 
     def spmm_cols(X):
         """K X, column by column, for a narrow (n, k) block (k <= 2)."""
@@ -550,7 +559,7 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     while it < maxiter and active.any():
         Z = precond.vcycle_multi(R32)
         steps.direction(Z, free, None if first else Qp, P, first)
-        _lib.call("fem_bsr_spmm", K64.mesh.ref, K64.vals.data_ptr(), P.data_ptr(), Q.data_ptr(), nb, st)
+        solvers.bsr_spmm(K64.mesh, K64.vals, P, Q, nb)
         rr = steps.update(P, Q, free, act_d, X, R, R32)
         Q, Qp = Qp, Q
         first = False

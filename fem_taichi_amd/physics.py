@@ -18,6 +18,7 @@ import torch
 
 from . import _lib
 from .groups import DEFAULT_BRICK, build_row_groups, build_tasks_torch, node_group_ids
+from .mesh import is_grid_mesh
 from .runtime import device, stream_ptr
 
 # Upper bound on the elements one row group may touch (LDS staging budget).
@@ -221,6 +222,13 @@ class Physics:
         if lds > 160 * 1024:
             raise ValueError(f"row groups need {lds} B of LDS; use a smaller brick")
         self.dev = DeviceMesh(coords, self.elem_nodes, pat, groups)
+        # node grid of the structured-grid kernels: x-fastest node ids and the
+        # element numbering of mesh.grid_connectivity (checked, not assumed)
+        self.dev.grid_nodes = None
+        if self.node_dims is not None:
+            dims = tuple(int(d) for d in self.node_dims)
+            if is_grid_mesh(self.elem_nodes, self.Num_Node, tuple(d - 1 for d in dims)):
+                self.dev.grid_nodes = dims
         self._symbolic = (coords, pat, ne_ptr, ne_list)
         self._kn_dev = None
         self._fields_dev = None

@@ -36,6 +36,19 @@ class SolveInfo:
     history: list
 
 
+def bsr_spmm(dev_mesh, vals, X, Y, nb):
+    """Y = K X for nb column-interleaved FP64 vectors: the structured-grid
+    16-column kernel (fem_bsr_spmm16_grid) when the mesh is a node grid of
+    that numbering and nb == 16, else fem_bsr_spmm."""
+    grid = getattr(dev_mesh, "grid_nodes", None)
+    if nb == 16 and grid is not None and X.data_ptr() % 16 == 0:
+        _lib.call("fem_bsr_spmm16_grid", dev_mesh.ref, grid[0], grid[1], grid[2], vals.data_ptr(), X.data_ptr(),
+                  Y.data_ptr(), stream_ptr())
+    else:
+        _lib.call("fem_bsr_spmm", dev_mesh.ref, vals.data_ptr(), X.data_ptr(), Y.data_ptr(), nb, stream_ptr())
+    return Y
+
+
 class BsrOperator:
     """A device BSR matrix on a mesh's pattern (f64 or f32 values)."""
 

@@ -225,6 +225,15 @@ int fem_bsr_spmv(const fem_mesh* mesh, const void* vals, int dtype, const void* 
 /* Y = A X for nrhs column-interleaved right-hand sides X[(3*i+p)*nrhs + c]. */
 int fem_bsr_spmm(const fem_mesh* mesh, const double* vals, const double* X, double* Y,
                  int64_t nrhs, void* stream);
+/* fem_bsr_spmm with 16 columns on a structured node grid of nx x ny x nz
+ * nodes (node id = ix + nx*(iy + ny*iz), the pattern's columns within one
+ * index step of the row along each axis, as for any hex8 grid of that
+ * numbering): a work-group walks up node planes of an 8 x 4 tile and keeps
+ * three planes of X in LDS, so X is fetched once per plane instead of
+ * gathered per block.  Deterministic; the sums run in a different order than
+ * fem_bsr_spmm's, so results agree to rounding.  X 16-byte aligned, Y != X. */
+int fem_bsr_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* vals,
+                        const double* X, double* Y, void* stream);
 /* y = M x with the scalar-per-block mass (the reference's CSR M). */
 int fem_mass_spmm(const fem_mesh* mesh, const double* m_blocks, const double* X, double* Y,
                   int64_t nrhs, void* stream);
@@ -314,6 +323,12 @@ int fem_dense_gemv_f32(const float* A, int64_t n, const float* x, float* y, void
  * vectors at once (the inner solves of block Lanczos). */
 int fem_mg_smooth_multi(const fem_mesh* mesh, const float* vals, const float* dinv, float omega,
                         const float* X, const float* B, float* OUT, int64_t nrhs, int mode, void* stream);
+/* fem_mg_smooth_multi's residual (mode 0) and update (mode 1) passes for 16
+ * columns on a structured node grid (the layout and conditions of
+ * fem_bsr_spmm16_grid). */
+int fem_mg_smooth16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const float* vals,
+                         const float* dinv, float omega, const float* X, const float* B, float* OUT, int mode,
+                         void* stream);
 /* Y = A X, A dense n x n FP32 row-major, X and Y n x nrhs row-major. */
 int fem_dense_gemm_nb_f32(const float* A, int64_t n, const float* X, float* Y, int64_t nrhs, void* stream);
 
