@@ -95,6 +95,7 @@ SIGNATURES = {
     "fem_bsr_spmm": (_I32, [_MP, _P, _P, _P, _I64, _P]),
     "fem_mass_spmm": (_I32, [_MP, _P, _P, _P, _I64, _P]),
     "fem_bsr_spmm16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, _P, _P]),
+    "fem_mass_spmm16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, _P, _P]),
     "fem_mg_smooth16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, ctypes.c_float, _P, _P, _P, _I32, _P]),
     "fem_bj_setup": (_I32, [_MP, _P, _I32, _P, _P, _P]),
     "fem_pcg_work_bytes": (_I64, [_I64]),

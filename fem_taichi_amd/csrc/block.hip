@@ -211,71 +211,109 @@ mpcg_direction_kernel(int64_t N, int nb, const double* __restrict__ Z, double* _
 // C (p x q) = A^T B for tall A (n x p, row stride lda) and B (n x q, row
 // stride ldb): the inner products of block Lanczos (SOLVE_eigen.py:56, 67,
 // 72-73, 76), whose reduction runs over the n = 3N rows.  Work-group g sums
-// its row chunk into a partial p x q block: tiles of R rows are staged in LDS
-// (R from a 48 KB budget); when p q <= 128 the 256 threads split into
-// G = 256 / (p q) row groups (thread t owns output t % pq, rows t / pq
-// (mod G)) folded in group order, else thread t owns outputs t, t + 256, ...
-// The partials are then summed per output in a fixed tree order, so the
-// result is deterministic.
+// its row chunk into a partial p x q block.  Tiles of R rows of A and B are
+// staged in LDS (coalesced, rows padded to whole 4-column groups, R from a
+// 48 KB budget); each thread owns a 4 x 4 block of outputs (4 values of A
+// and 4 of B per row feed 16 fused multiply-adds) and, when the p q / 16
+// output blocks leave threads over, the threads split into G row groups
+// (rows r = group (mod G)) that are folded in group order at the end.  The
+// partials are then summed per output in a fixed tree order, so the result
+// is deterministic.
 constexpr int kTT = 256;
 constexpr int kTnLds = 6144;          // doubles of LDS for one A/B row tile
 constexpr int kGridTN = 1024;
 
-template <int NACC>
+// Rows rb.. of a tall matrix (w used columns, row stride ld) into LDS rows of
+// W doubles (zero padded), total = rows * W entries: every load of a batch is
+// issued before the batch's LDS stores, so a thread has kTnBatch loads in
+// flight instead of one.
+constexpr int kTnBatch = 8;
+__device__ __forceinline__ void tn_stage(double* __restrict__ dst, const double* __restrict__ src, int64_t ld, int w,
+                                         int W, int64_t rb, int total, int tid) {
+  for (int e0 = tid; e0 < total; e0 += kTnBatch * kTT) {
+    double v[kTnBatch];
+#pragma unroll
+    for (int u = 0; u < kTnBatch; ++u) {
+      const int e = e0 + u * kTT;
+      const int r = e / W, c = e - r * W;
+      v[u] = (e < total && c < w) ? src[(rb + r) * ld + c] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kTnBatch; ++u) {
+      const int e = e0 + u * kTT;
+      if (e < total) dst[e] = v[u];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(kTT)
 gemm_tn_partial_kernel(const double* __restrict__ A, int64_t lda, int64_t n, int p, const double* __restrict__ B,
-                       int64_t ldb, int q, int R, int G, double* __restrict__ part) {
-  extern __shared__ double sm[];
-  double* sA = sm;                      // R x p
-  double* sB = sm + R * p;              // R x q
-  const int pq = p * q;
+                       int64_t ldb, int q, int R, double* __restrict__ part) {
+  extern __shared__ __align__(16) double sm[];
+  const int pt = (p + 3) / 4, qt = (q + 3) / 4, ntile = pt * qt;
+  const int PA = 4 * pt, QB = 4 * qt;
+  const int G = ntile <= kTT ? kTT / ntile : 1;
+  double* sA = sm;                      // R x PA
+  double* sB = sm + R * PA;             // R x QB
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t r0 = (int64_t)blockIdx.x * chunk;
   const int64_t r1 = r0 + chunk < n ? r0 + chunk : n;
-  const int grp = G > 1 ? (int)threadIdx.x / pq : 0;
-  double acc[NACC];
-  int oi[NACC], oj[NACC];
+  const int tid = threadIdx.x;
+  const int grp = tid / ntile;          // >= G: idle (or, when ntile > 256, a second pass below)
+  double acc[4][4];
+  for (int t0 = 0; t0 < ntile; t0 += kTT) {         // more than 256 output blocks: passes over the rows
+    const int tile = t0 + (G > 1 ? tid % ntile : tid);
+    const bool on = G > 1 ? grp < G : tile < ntile;
+    const int ti = on ? tile / qt : 0, tj = on ? tile % qt : 0;
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) {
-    acc[a] = 0.0;
-    const int o = G > 1 ? (int)threadIdx.x % pq : (int)threadIdx.x + a * kTT;
-    oi[a] = o < pq ? o / q : 0;
-    oj[a] = o < pq ? o % q : 0;
-  }
-  for (int64_t rb = r0; rb < r1; rb += R) {
-    const int nr = (int)(r1 - rb < R ? r1 - rb : R);
-    __syncthreads();
-    for (int e = threadIdx.x; e < nr * p; e += kTT) {
-      const int r = e / p;
-      sA[e] = A[(rb + r) * lda + (e - r * p)];
-    }
-    for (int e = threadIdx.x; e < nr * q; e += kTT) {
-      const int r = e / q;
-      sB[e] = B[(rb + r) * ldb + (e - r * q)];
-    }
-    __syncthreads();
-    if (grp < G) {
-      for (int r = grp; r < nr; r += G) {
+    for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int a = 0; a < NACC; ++a) acc[a] += sA[r * p + oi[a]] * sB[r * q + oj[a]];
+      for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+    for (int64_t rb = r0; rb < r1; rb += R) {
+      const int nr = (int)(r1 - rb < R ? r1 - rb : R);
+      __syncthreads();
+      tn_stage(sA, A, lda, p, PA, rb, nr * PA, tid);
+      tn_stage(sB, B, ldb, q, QB, rb, nr * QB, tid);
+      __syncthreads();
+      if (on) {
+        for (int r = G > 1 ? grp : 0; r < nr; r += G > 1 ? G : 1) {
+          const double2* pa = reinterpret_cast<const double2*>(sA + r * PA + 4 * ti);
+          const double2* pb = reinterpret_cast<const double2*>(sB + r * QB + 4 * tj);
+          const double2 a01 = pa[0], a23 = pa[1], b01 = pb[0], b23 = pb[1];
+          const double av[4] = {a01.x, a01.y, a23.x, a23.y}, bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+          for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = fma(av[a], bv[b], acc[a][b]);
+        }
       }
     }
-  }
-  if (G > 1) {
+    // fold the row groups in order, then store the block's partial outputs
     __syncthreads();
-    sm[threadIdx.x] = grp < G ? acc[0] : 0.0;
-    __syncthreads();
-    if ((int)threadIdx.x < pq) {
-      double s = 0.0;
-      for (int g = 0; g < G; ++g) s += sm[g * pq + threadIdx.x];
-      part[(int64_t)blockIdx.x * pq + threadIdx.x] = s;
-    }
-    return;
-  }
+    if (on && G > 1) {
 #pragma unroll
-  for (int a = 0; a < NACC; ++a) {
-    const int o = threadIdx.x + a * kTT;
-    if (o < pq) part[(int64_t)blockIdx.x * pq + o] = acc[a];
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) sm[(grp * ntile + tile) * 16 + 4 * a + b] = acc[a][b];
+    }
+    __syncthreads();
+    if (G > 1) {
+      for (int o = tid; o < ntile * 16; o += kTT) {
+        double v = 0.0;
+        for (int g = 0; g < G; ++g) v += sm[g * ntile * 16 + o];
+        const int tl = o / 16, ab = o % 16;
+        const int i = (tl / qt) * 4 + ab / 4, jj = (tl % qt) * 4 + ab % 4;
+        if (i < p && jj < q) part[(int64_t)blockIdx.x * p * q + i * q + jj] = v;
+      }
+    } else if (on) {
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const int i = 4 * ti + a, jj = 4 * tj + b;
+          if (i < p && jj < q) part[(int64_t)blockIdx.x * p * q + i * q + jj] = acc[a][b];
+        }
+    }
   }
 }
 
@@ -534,21 +572,12 @@ namespace {
 int gemm_tn_impl(const double* A, int64_t lda, const double* B, int64_t ldb, int64_t n, int64_t p, int64_t q,
                  double* C, double* part, hipStream_t st) {
   const int pi = (int)p, qi = (int)q, pqi = (int)(p * q);
-  const int G = pqi <= kTT / 2 ? kTT / pqi : 1;
-  const int nacc = G > 1 ? 1 : (pqi + kTT - 1) / kTT;
-  int R = kTnLds / (pi + qi);
+  const int PA = 4 * ((pi + 3) / 4), QB = 4 * ((qi + 3) / 4);
+  int R = kTnLds / (PA + QB);
   if (R > 256) R = 256;
-  const size_t lds = (size_t)R * (pi + qi) * 8;
+  const size_t lds = (size_t)kTnLds * 8;      // also holds the row-group fold (<= 4096 doubles)
   const int nblk = n < 1 ? 1 : (int)(n < kGridTN ? n : kGridTN);
-#define FEM_TN(NA) \
-  gemm_tn_partial_kernel<NA><<<nblk, kTT, lds, st>>>(A, lda, n, pi, B, ldb, qi, R, G, part)
-  if (nacc <= 1) FEM_TN(1);
-  else if (nacc <= 2) FEM_TN(2);
-  else if (nacc <= 4) FEM_TN(4);
-  else if (nacc <= 8) FEM_TN(8);
-  else if (nacc <= 16) FEM_TN(16);
-  else FEM_TN(32);
-#undef FEM_TN
+  gemm_tn_partial_kernel<<<nblk, kTT, lds, st>>>(A, lda, n, pi, B, ldb, qi, R, part);
   FEM_LAUNCH_CHECK();
   gemm_tn_sum_kernel<<<pqi, kTT, 0, st>>>(part, pqi, nblk, C);
   FEM_LAUNCH_CHECK();
@@ -704,12 +733,18 @@ gemm_nn_kernel(const double* __restrict__ A, int64_t lda, const double* __restri
   for (int j0 = 0; j0 < k; j0 += kNnChunk) {
     const int kc = k - j0 < kNnChunk ? k - j0 : kNnChunk;
     __syncthreads();
-#pragma unroll 4
+    double v[kNnChunk];                           // all of the thread's loads in flight, then the stores
+#pragma unroll
     for (int i = 0; i < kNnChunk; ++i) {
       const int e = tid + i * kNnT;               // row e / 16, column e % 16 of the chunk
       const int r = e / kNnChunk, j = e - r * kNnChunk;
       const int64_t row = row0 + r;
-      sA[r * (kNnChunk + 1) + j] = (row < n && j < kc) ? A[row * lda + j0 + j] : 0.0;
+      v[i] = (row < n && j < kc) ? A[row * lda + j0 + j] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < kNnChunk; ++i) {
+      const int e = tid + i * kNnT;
+      sA[(e / kNnChunk) * (kNnChunk + 1) + e % kNnChunk] = v[i];
     }
     for (int e = tid; e < kNnChunk * QMAX; e += kNnT) {
       const int j = e / QMAX, c = e - j * QMAX;

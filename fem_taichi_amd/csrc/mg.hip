@@ -249,10 +249,16 @@ smooth_multi16_kernel(fem_mesh m, const float* __restrict__ vals, const float* _
   for (int c0 = 0; c0 < co; c0 += kSm16Team) {
     const int nblk = min(kSm16Team, co - c0);
     const float* src = vals + 9 * (sm + c0);
+    float kv[9];                    // the nine loads in flight together, then the LDS stores
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int d = j + kSm16Team * k;
-      if (d < 9 * nblk) s[d] = src[d];
+      kv[k] = d < 9 * nblk ? src[d] : float(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int d = j + kSm16Team * k;
+      if (d < 9 * nblk) s[d] = kv[k];
     }
     const int colj = j < nblk ? m.re_node[sm + c0 + j] : 0;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

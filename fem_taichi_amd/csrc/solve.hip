@@ -199,10 +199,16 @@ spmm16_kernel(fem_mesh m, const double* __restrict__ vals, const double* __restr
   for (int c0 = 0; c0 < co; c0 += kS16Team) {
     const int nblk = min(kS16Team, co - c0);
     const double* src = vals + 9 * (sm + c0);
+    double kv[9];                    // the nine loads in flight together, then the LDS stores
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const int d = j + kS16Team * k;
-      if (d < 9 * nblk) s[d] = src[d];
+      kv[k] = d < 9 * nblk ? src[d] : double(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int d = j + kS16Team * k;
+      if (d < 9 * nblk) s[d] = kv[k];
     }
     const int colj = j < nblk ? m.re_node[sm + c0 + j] : 0;
     wave_lds_sync();

@@ -70,7 +70,9 @@ __device__ __forceinline__ void st_stage(T (&a)[3][kNb], bool upper) {
 }
 
 // MODE 0: OUT = B - K X;  1: OUT = X + w Dinv (B - K X);  3: OUT = K X.
-template <typename T, int MODE>
+// BW: values per block, 9 (3 x 3 blocks of K) or 1 (the mass matrix's one
+// scalar per node pair on each DOF diagonal).
+template <typename T, int MODE, int BW = 9>
 __global__ void __launch_bounds__(kSt, sizeof(T) == 8 ? 2 : 3)
 stencil16_kernel(const int32_t* __restrict__ sm_node, const int32_t* __restrict__ co_node,
                  const int32_t* __restrict__ re_node, const T* __restrict__ vals, const T* __restrict__ X,
@@ -152,10 +154,10 @@ stencil16_kernel(const int32_t* __restrict__ sm_node, const int32_t* __restrict_
           const int rem = col - cz * plane32;
           const int cy = rem / NX, cx = rem - cy * NX;
           const T* xs = ring + ((cz % 3) * kStNP + (cy - y0 + 1) * kStPX + (cx - x0 + 1)) * S;
-          const T* kv = vals + 9 * (sm + b);
+          const T* kv = vals + BW * (sm + b);
           T kb[9];
 #pragma unroll
-          for (int t = 0; t < 9; ++t) kb[t] = kv[t];
+          for (int t = 0; t < 9; ++t) kb[t] = BW == 9 ? kv[t] : (t % 4 == 0 ? kv[0] : T(0));
 #pragma unroll
           for (int c = 0; c < kNb; c += CW) {
             const V v0 = *reinterpret_cast<const V*>(xs + c);
@@ -165,10 +167,18 @@ stencil16_kernel(const int32_t* __restrict__ sm_node, const int32_t* __restrict_
             const T* e1 = reinterpret_cast<const T*>(&v1);
             const T* e2 = reinterpret_cast<const T*>(&v2);
 #pragma unroll
-            for (int u = 0; u < CW; ++u)
+            for (int u = 0; u < CW; ++u) {
+              if (BW == 9) {
 #pragma unroll
-              for (int p = 0; p < 3; ++p)
-                acc[p][c + u] = fma(kb[3 * p], e0[u], fma(kb[3 * p + 1], e1[u], fma(kb[3 * p + 2], e2[u], acc[p][c + u])));
+                for (int p = 0; p < 3; ++p)
+                  acc[p][c + u] =
+                      fma(kb[3 * p], e0[u], fma(kb[3 * p + 1], e1[u], fma(kb[3 * p + 2], e2[u], acc[p][c + u])));
+              } else {
+                acc[0][c + u] = fma(kb[0], e0[u], acc[0][c + u]);
+                acc[1][c + u] = fma(kb[0], e1[u], acc[1][c + u]);
+                acc[2][c + u] = fma(kb[0], e2[u], acc[2][c + u]);
+              }
+            }
           }
         }
       }
@@ -230,11 +240,11 @@ int st_cu_count() {
   return g_st_cus[dev];
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int BW = 9>
 int launch_stencil16(const fem_mesh& m, int64_t nx, int64_t ny, int64_t nz, const T* vals, const T* X, const T* B,
                      const T* dinv, T w, T* OUT, hipStream_t st) {
   constexpr int lds = st_lds_bytes<T>();
-  auto kern = stencil16_kernel<T, MODE>;
+  auto kern = stencil16_kernel<T, MODE, BW>;
   FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
   const int ntx = (int)ceil_div(nx, kStTX), nty = (int)ceil_div(ny, kStTY);
@@ -273,6 +283,15 @@ extern "C" int fem_bsr_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny,
   FEM_REQUIRE(vals && X && Y && X != Y, FEM_E_INVALID, "null pointer, or Y aliases X");
   FEM_REQUIRE(reinterpret_cast<uintptr_t>(X) % 16 == 0, FEM_E_INVALID, "X must be 16-byte aligned");
   return launch_stencil16<double, 3>(*mesh, nx, ny, nz, vals, X, nullptr, nullptr, 0.0, Y, as_stream(stream));
+}
+
+extern "C" int fem_mass_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz,
+                                    const double* m_blocks, const double* X, double* Y, void* stream) {
+  int rc = check_grid16(mesh, nx, ny, nz);
+  if (rc) return rc;
+  FEM_REQUIRE(m_blocks && X && Y && X != Y, FEM_E_INVALID, "null pointer, or Y aliases X");
+  FEM_REQUIRE(reinterpret_cast<uintptr_t>(X) % 16 == 0, FEM_E_INVALID, "X must be 16-byte aligned");
+  return launch_stencil16<double, 3, 1>(*mesh, nx, ny, nz, m_blocks, X, nullptr, nullptr, 0.0, Y, as_stream(stream));
 }
 
 extern "C" int fem_mg_smooth16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const float* vals,

@@ -343,11 +343,17 @@ class Eigen_solver:
         if inner not in ("mg", "pcg"):
             raise ValueError(f"unknown inner solver {inner!r}")
 
+        grid = getattr(dev_mesh, "grid_nodes", None)
+
         def m_apply(X):
             X = X.contiguous()
             Y = torch.empty(X.shape, dtype=torch.float64, device=X.device)
-            _lib.call("fem_mass_spmm", dev_mesh.ref, mb.data_ptr(), X.data_ptr(), Y.data_ptr(), X.shape[1],
-                      stream_ptr())
+            if X.shape[1] == 16 and grid is not None and X.data_ptr() % 16 == 0:
+                _lib.call("fem_mass_spmm16_grid", dev_mesh.ref, grid[0], grid[1], grid[2], mb.data_ptr(),
+                          X.data_ptr(), Y.data_ptr(), stream_ptr())
+            else:
+                _lib.call("fem_mass_spmm", dev_mesh.ref, mb.data_ptr(), X.data_ptr(), Y.data_ptr(), X.shape[1],
+                          stream_ptr())
             return Y
 
         self.ops = MultigridBlockOps(Mo, m_apply, k_vals=kv) if inner == "mg" else BlockOps(dev_mesh, kv, m_apply)

@@ -234,6 +234,10 @@ int fem_bsr_spmm(const fem_mesh* mesh, const double* vals, const double* X, doub
  * fem_bsr_spmm's, so results agree to rounding.  X 16-byte aligned, Y != X. */
 int fem_bsr_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* vals,
                         const double* X, double* Y, void* stream);
+/* fem_mass_spmm with 16 columns on a structured node grid (the layout and
+ * conditions of fem_bsr_spmm16_grid). */
+int fem_mass_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* m_blocks,
+                         const double* X, double* Y, void* stream);
 /* y = M x with the scalar-per-block mass (the reference's CSR M). */
 int fem_mass_spmm(const fem_mesh* mesh, const double* m_blocks, const double* X, double* Y,
                   int64_t nrhs, void* stream);
