@@ -131,6 +131,7 @@ SIGNATURES = {
     "fem_fcg_work_bytes": (_I64, [_I64]),
     "fem_fcg_direction": (_I32, [_I64, _I64, _P, _I32, _P, _P, _P, _I32, _P, _I64, _P]),
     "fem_fcg_update": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+    "fem_fcg_residual": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
 }
 
 _lib = None

@@ -512,6 +512,36 @@ fcg_update_kernel(int64_t n, int nb, const double* __restrict__ P, const double*
   fcg_block_fold(rr, f, nb, red, part + (int64_t)blockIdx.x * nb);
 }
 
+// t = (B - Y) * mask_i (Y NULL: t = B * mask_i; mask NULL: all ones);
+// partials of sum t^2 and sum B^2 per column; columns with sel[c] != 0 (all
+// when sel is NULL) also get R = t and R32 = (float)t.
+__global__ void __launch_bounds__(kFT)
+fcg_residual_kernel(int64_t n, int nb, const double* __restrict__ B, const double* __restrict__ Y,
+                    const double* __restrict__ mask, const int32_t* __restrict__ sel, double* __restrict__ R,
+                    float* __restrict__ R32, double* __restrict__ part_t, double* __restrict__ part_b) {
+  __shared__ double red[kFT];
+  const FcgMap f = fcg_map(n, nb);
+  double tt = 0.0, bb = 0.0;
+  if (f.on) {
+    const bool wr = sel == nullptr || sel[f.c] != 0;
+#pragma unroll 4
+    for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+      const int64_t e = i * nb + f.c;
+      const double b = B[e];
+      double t = Y ? b - Y[e] : b;
+      if (mask) t *= mask[i];
+      tt += t * t;
+      bb += b * b;
+      if (wr) {
+        if (R) R[e] = t;
+        if (R32) R32[e] = (float)t;
+      }
+    }
+  }
+  fcg_block_fold(tt, f, nb, red, part_t + (int64_t)blockIdx.x * nb);
+  fcg_block_fold(bb, f, nb, red, part_b + (int64_t)blockIdx.x * nb);
+}
+
 int fcg_grid(int64_t n) { return (int)(n < kFcgGrid ? (n < 1 ? 1 : n) : kFcgGrid); }
 
 int check_nb(int64_t nb) {
@@ -701,6 +731,28 @@ extern "C" int fem_fcg_update(int64_t n, int64_t nb, const double* P, const doub
   fcg_update_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, mask, sc, X, R, R32, part3);
   FEM_LAUNCH_CHECK();
   fcg_sum_kernel<<<1, kFoldT, 0, st>>>(part3, grid, nbi, rr);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_fcg_residual(int64_t n, int64_t nb, const double* B, const double* Y, const double* mask,
+                                const int32_t* sel, double* R, float* R32, double* out, void* work,
+                                int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64, FEM_E_INVALID, "nb must be in [1, 64] (got %lld)", (long long)nb);
+  FEM_REQUIRE(n >= 0, FEM_E_INVALID, "negative row count");
+  FEM_REQUIRE(work && work_bytes >= fem_fcg_work_bytes(nb), FEM_E_INVALID, "work too small");
+  FEM_REQUIRE(out, FEM_E_INVALID, "null out");
+  FEM_REQUIRE(n == 0 || B, FEM_E_INVALID, "null B");
+  FEM_REQUIRE(!R || (R != B && R != Y), FEM_E_INVALID, "R must not alias B or Y");
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  double* part2 = part + (int64_t)kFcgGrid * nb;   // the first two partial arrays; the carried scalars stay
+  const int grid = fcg_grid(n), nbi = (int)nb;
+  fcg_residual_kernel<<<grid, kFT, 0, st>>>(n, nbi, B, Y, mask, sel, R, R32, part, part2);
+  FEM_LAUNCH_CHECK();
+  fcg_sum_kernel<<<1, kFoldT, 0, st>>>(part, grid, nbi, out);
+  FEM_LAUNCH_CHECK();
+  fcg_sum_kernel<<<1, kFoldT, 0, st>>>(part2, grid, nbi, out + nb);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }

@@ -24,16 +24,9 @@ import torch
 
 from . import _lib
 from .runtime import device, stream_ptr
+from .runtime import to_host as _to_host
 
 PENALTY = 1e20   # FEM_model_csr.py:241
-
-
-def _to_host(t):
-    """A device tensor as a numpy array, copied through a pinned buffer (about
-    three times the rate of a copy into pageable memory)."""
-    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
-    h.copy_(t)
-    return h.numpy()
 
 
 def stage_bounds(Bounds):

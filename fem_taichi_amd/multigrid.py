@@ -396,6 +396,17 @@ class FcgSteps:
                   stream_ptr())
         return self.rr
 
+    def residual(self, B, Y, mask, sel, R, R32):
+        """t = (B - Y) mask (Y None: B mask); R = t and R32 = (float) t on the
+        columns with sel[c] != 0 (sel None: all; R, R32 may be None).
+        Returns (|t_c|, |B_c|) per column as numpy arrays (one pass)."""
+        self._check(B, Y, R, R32)
+        out = torch.empty(2 * self.nb, dtype=torch.float64, device=B.device)
+        _lib.call("fem_fcg_residual", self.n, self.nb, B.data_ptr(), _ptr(Y), _ptr(mask), _ptr(sel), _ptr(R),
+                  _ptr(R32), out.data_ptr(), self.work.data_ptr(), self.wb, stream_ptr())
+        sq = np.sqrt(np.maximum(out.cpu().numpy(), 0.0))
+        return sq[:self.nb], sq[self.nb:]
+
 
 def _ptr(t):
     return None if t is None else t.data_ptr()
@@ -491,16 +502,6 @@ def solve_fcg_mg(model, K64, F, precond, tol=1e-8, maxiter=1000, check_every=1, 
                "trace": trace}
 
 
-def _colnorms(A):
-    """Column 2-norms of a tall (n, k) FP64 device block, as numpy: the
-    diagonal of A^T A from the deterministic row-split product (a library
-    reduction along the rows of a narrow row-major block is several times
-    slower)."""
-    from .solve_eigen import gram
-    A = A.contiguous()
-    return np.sqrt(np.maximum(torch.diagonal(gram(A, A)).cpu().numpy(), 0.0))
-
-
 def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None, dinv64=None, patience=3,
                        drop=1e-2, check_after=50, floor=1e-7):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
@@ -523,35 +524,28 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     n, nb = B.shape
     if free is None:
         free = free_dof_mask(model)
-    freec = free[:, None]
     fixed = 1.0 - free
     if dinv64 is None:
         dinv64, _ = K64.block_jacobi()
     d3 = dinv64.view(-1, 3, 3)
+    Y = torch.empty_like(B)
 
     def spmm(X):
-        X = X.contiguous()
-        Y = torch.empty_like(X)
         return solvers.bsr_spmm(K64.mesh, K64.vals, X, Y, nb)
 
-    def spmm_cols(X):
-        """K X, column by column, for a narrow (n, k) block (k <= 2)."""
-        return torch.stack([K64.matvec(X[:, j].contiguous()) for j in range(X.shape[1])], dim=1)
-
     B = B.contiguous()
-    bn = _colnorms(B)
+    steps = FcgSteps(n, nb, B.device)
     X = torch.zeros_like(B)
-    R = B * freec
-    R32 = R.float()
+    R = torch.empty_like(B)
+    R32 = torch.empty(B.shape, dtype=torch.float32, device=B.device)
+    anchor, bn = steps.residual(B, None, free, None, R, R32)      # R = B masked to the free DOFs
     active = bn > 0
     converged = ~active
     stagnated = np.zeros(nb, dtype=bool)
-    anchor = _colnorms(R)
     best = np.full(nb, np.inf)
     stalls = np.zeros(nb, dtype=int)
     last_check = np.zeros(nb, dtype=int)
     act_d = torch.as_tensor(active.astype(np.int32), device=B.device)
-    steps = FcgSteps(n, nb, B.device)
     P = torch.empty_like(B)
     Q, Qp = torch.empty_like(B), torch.empty_like(B)
     first = True
@@ -569,22 +563,11 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
             break
         need = active & ((rf < drop * anchor) | (rf < tol * bn) | (it - last_check >= check_after))
         if need.any():
-            # true residuals of the columns that asked for one (one or two
-            # of them by single-vector products)
+            # true residuals of the columns that asked for one replace their
+            # recursive ones (one product of all columns, one fused pass)
             cols = np.nonzero(need)[0]
-            tn = np.zeros(nb)
-            if len(cols) > 2:           # one product of all columns costs about as much as two SpMVs
-                Rt = (B - spmm(X)) * freec
-                tn[cols] = _colnorms(Rt)[cols]
-                sel = torch.as_tensor(need, device=B.device)
-                R.copy_(torch.where(sel[None, :], Rt, R))
-                R32.copy_(R)
-            else:
-                ci = torch.as_tensor(cols, device=B.device)
-                Rt = (B[:, ci] - spmm_cols(X[:, ci].contiguous())) * freec
-                tn[cols] = _colnorms(Rt)
-                R[:, ci] = Rt
-                R32[:, ci] = Rt.float()
+            sel = torch.as_tensor(need.astype(np.int32)).to(B.device)
+            tn, _ = steps.residual(B, spmm(X), free, sel, R, R32)
             for c in cols:
                 anchor[c] = tn[c]
                 last_check[c] = it
@@ -599,10 +582,15 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
                         stagnated[c] = True
                         active[c] = False
             act_d.copy_(torch.as_tensor(active.astype(np.int32)))
-    # the penalised DOFs from their own rows
-    Rp = (B - spmm(X)) * fixed[:, None]
-    X.add_(torch.bmm(d3, Rp.view(-1, 3, nb)).view(n, nb) * fixed[:, None])
-    res = _colnorms(B - spmm(X)) / np.where(bn > 0, bn, 1.0)
+    # the penalised DOFs from their own rows (the nodes that have one)
+    fn = torch.nonzero(fixed.view(-1, 3).amax(dim=1) > 0).flatten()
+    if fn.numel():
+        spmm(X)
+        fm = fixed.view(-1, 3)[fn][:, :, None]
+        Rp = (B.view(-1, 3, nb)[fn] - Y.view(-1, 3, nb)[fn]) * fm
+        X.view(-1, 3, nb).index_add_(0, fn, (d3[fn][:, :, :, None] * Rp[:, None, :, :]).sum(dim=2) * fm)
+    tn, _ = steps.residual(B, spmm(X), None, None, None, None)
+    res = tn / np.where(bn > 0, bn, 1.0)
     converged = converged & (res < tol)
     return X, {"iterations": it, "rel_residual": res, "converged": converged, "stagnated": stagnated}
 

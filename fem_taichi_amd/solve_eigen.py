@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .runtime import device, stream_ptr
+from .runtime import device, stream_ptr, to_host
 
 
 def _pow2(n):
@@ -290,11 +290,13 @@ def lanczos(ops, num_eigen=100, num_block=10, seed=0, inner_tol=1e-12, u0=None):
     theta, V = torch.linalg.eigh(T / scale)
     theta = theta * scale
     lam = 1.0 / theta
-    X = torch.empty(n, m, dtype=torch.float64, device=d)
-    for c0 in range(0, m, 64):                                 # Ritz vectors Q V
-        gemm_nn(Q, V[:, c0:c0 + 64], X[:, c0:c0 + 64])
-    ind = torch.argsort(lam)
-    return lam[ind][:num_eigen], X[:, ind][:, :num_eigen], T
+    ind = torch.argsort(lam)[:num_eigen]
+    Vs = V[:, ind]                                             # the returned Ritz pairs, ascending
+    k = Vs.shape[1]
+    X = torch.empty(n, k, dtype=torch.float64, device=d)
+    for c0 in range(0, k, 64):                                 # Ritz vectors Q V
+        gemm_nn(Q, Vs[:, c0:c0 + 64], X[:, c0:c0 + 64])
+    return lam[ind], X, T
 
 
 class Eigen_solver:
@@ -363,7 +365,7 @@ class Eigen_solver:
         if verbose:
             for i in range(num_eigen):
                 print('\t\t\tmode {:3d} frequency {:.3e}'.format(i + 1, float(freq[i])))
-        return freq.cpu().numpy(), X.cpu().numpy()
+        return freq.cpu().numpy(), to_host(X)
 
 
 class Block_lancozs:
@@ -397,4 +399,4 @@ class Block_lancozs:
 
         ops = BlockOps(kmesh, kv, m_apply)
         lam, X, _ = lanczos(ops, self.num_eigen, self.num_block, seed=seed, inner_tol=inner_tol, u0=u0)
-        return lam.cpu().numpy(), X.cpu().numpy()
+        return lam.cpu().numpy(), to_host(X)

@@ -26,6 +26,7 @@ from __future__ import annotations
 import torch
 
 from . import solvers
+from .runtime import to_host
 
 
 class Static_solver:
@@ -60,7 +61,7 @@ class Static_solver:
         if verbose:
             print(f"\t\t\t{info.method}: {info.iterations} iterations, relative residual "
                   f"{info.rel_residual:.3e}, {info.seconds:.3f} s")
-        return U.cpu().numpy()
+        return to_host(U)
 
     def _solve_newton(self, max_errs, max_iters, tol, verbose):
         M = self.Model
@@ -89,4 +90,4 @@ class Static_solver:
         # reference's Eigen_solver factors Model.KT (SOLVE_eigen.py:32)
         M.KT_device = kt
         self.info = {"steps": step, "residual": self.history[-1]}
-        return U.cpu().numpy()
+        return to_host(U)

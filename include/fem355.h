@@ -398,6 +398,16 @@ int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dtype, const d
 int fem_fcg_update(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
                    const int32_t* active, double* X, double* R, float* R32, double* rr, void* work,
                    int64_t work_bytes, void* stream);
+/* The residual of the block solve (multigrid.py solve_block_fcg_mg's start,
+ * true-residual checks and final norms, in one pass; no counterpart kernel in
+ * the reference): t = (B - Y) * mask (Y NULL: B * mask; mask NULL: all
+ * ones); out[c] = |t_c|^2 and out[nb + c] = |B_c|^2 (device array of 2 nb);
+ * columns with sel[c] != 0 (sel NULL: all) also get R = t and R32 =
+ * (float)t (either may be NULL).  Uses the work buffer's partials only, so
+ * the scalars fem_fcg_update carries to fem_fcg_direction survive. */
+int fem_fcg_residual(int64_t n, int64_t nb, const double* B, const double* Y, const double* mask,
+                     const int32_t* sel, double* R, float* R32, double* out, void* work, int64_t work_bytes,
+                     void* stream);
 
 /* ---- 3x3x3 mesh convolution on the structured node grid (the learned
  *      preconditioner; no counterpart in the reference) ---------------------

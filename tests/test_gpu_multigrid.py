@@ -285,3 +285,43 @@ def test_fcg_steps_match_numpy(gpu, n, nb):
     again = run()
     for a, b in zip(again, (P, X, R, rr, R32)):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,nb", [(5, 1), (77777, 1), (30000, 8), (20000, 16), (300, 64)])
+def test_fcg_residual_matches_numpy(gpu, n, nb):
+    """fem_fcg_residual (through FcgSteps.residual): t = (B - Y) mask, its
+    column norms and those of B against numpy; R and R32 written on the
+    selected columns only (the others keep their contents), bit-exact there
+    (one subtraction and one product, as numpy does them); Y, mask, sel and
+    R/R32 each absent; the carried FCG scalars in the shared work buffer are
+    left alone."""
+    import torch
+    from fem_taichi_amd import multigrid as MG
+    print(f"\nfcg residual n {n} nb {nb}", flush=True)
+    rng = np.random.default_rng(3 * n + nb)
+    B, Y = rng.standard_normal((n, nb)), rng.standard_normal((n, nb))
+    mask = (rng.random(n) > 0.2).astype(np.float64)
+    sel = (rng.random(nb) > 0.5).astype(np.int32)
+    sel[0] = 1
+    c = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+    st = MG.FcgSteps(n, nb, "cuda")
+    st.work[-4096:] = 7                                  # the carried scalars' region: must survive
+    keep = st.work[-4096:].clone()
+    R = torch.full((n, nb), -1.0, dtype=torch.float64, device="cuda")
+    R32 = torch.full((n, nb), -1.0, dtype=torch.float32, device="cuda")
+    tn, bn = st.residual(c(B), c(Y), c(mask), c(sel), R, R32)
+    t = (B - Y) * mask[:, None]
+    assert np.allclose(tn, np.linalg.norm(t, axis=0), rtol=1e-13, atol=0)
+    assert np.allclose(bn, np.linalg.norm(B, axis=0), rtol=1e-13, atol=0)
+    Rh, R32h = R.cpu().numpy(), R32.cpu().numpy()
+    on = sel > 0
+    assert np.array_equal(Rh[:, on], t[:, on]) and np.array_equal(R32h[:, on], t[:, on].astype(np.float32))
+    assert np.all(Rh[:, ~on] == -1.0) and np.all(R32h[:, ~on] == -1.0)
+    assert torch.equal(st.work[-4096:], keep)
+    tn2, _ = st.residual(c(B), None, None, None, R, None)          # R = B
+    assert np.allclose(tn2, np.linalg.norm(B, axis=0), rtol=1e-13, atol=0)
+    assert np.array_equal(R.cpu().numpy(), B)
+    tn3, bn3 = st.residual(c(B), c(Y), None, None, None, None)     # norms only
+    assert np.allclose(tn3, np.linalg.norm(B - Y, axis=0), rtol=1e-13, atol=0)
+    again = st.residual(c(B), c(Y), c(mask), c(sel), R, R32)
+    assert np.array_equal(again[0], tn) and np.array_equal(again[1], bn)
