@@ -240,6 +240,13 @@ class MultigridPreconditioner:
         self.x = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
         self.t = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
         self.applications = 0
+        # damped block-Jacobi sweeps before and after the coarse correction:
+        # one for a single vector (the static solve: two take fewer
+        # iterations, no less time), two for the 16-column inner solves of
+        # block Lanczos (30% fewer iterations and 14% less time than one on
+        # the 500 x 60 x 75 blade; three or four take no less time)
+        self.sweeps = 1
+        self.sweeps_multi = 2
         self._multi = {}
         torch.cuda.synchronize()
         self.setup_seconds = time.perf_counter() - t0
@@ -297,6 +304,10 @@ class MultigridPreconditioner:
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_smooth", dev.ref, None, _lib.LAYOUT_BSR, dv, w, None, self.b[l].data_ptr(),
                       self.x[l].data_ptr(), 2, st)
+            for _ in range(self.sweeps - 1):
+                _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(),
+                          self.b[l].data_ptr(), self.t[l].data_ptr(), 1, st)
+                self.x[l], self.t[l] = self.t[l], self.x[l]
             _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
                       self.t[l].data_ptr(), 0, st)
             _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, self.t[l].data_ptr(), self.b[l + 1].data_ptr(), 1,
@@ -309,9 +320,10 @@ class MultigridPreconditioner:
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, self.x[l + 1].data_ptr(), self.x[l].data_ptr(),
                       1, st)
-            _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(), self.b[l].data_ptr(),
-                      self.t[l].data_ptr(), 1, st)
-            self.x[l], self.t[l] = self.t[l], self.x[l]
+            for _ in range(self.sweeps):
+                _lib.call("fem_mg_smooth", dev.ref, k, _lib.LAYOUT_BSR, dv, w, self.x[l].data_ptr(),
+                          self.b[l].data_ptr(), self.t[l].data_ptr(), 1, st)
+                self.x[l], self.t[l] = self.t[l], self.x[l]
         return self.x[0]
 
     def __call__(self, r: torch.Tensor) -> torch.Tensor:
@@ -335,6 +347,9 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_smooth_multi", dev.ref, None, dv, w, None, b[l].data_ptr(), x[l].data_ptr(), nb, 2, st)
+            for _ in range(self.sweeps_multi - 1):
+                _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
+                x[l], t[l] = t[l], x[l]
             _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 0, st)
             _lib.call("fem_mg_restrict", self.hier.transfers[l].ref, t[l].data_ptr(), b[l + 1].data_ptr(), nb, st)
         _lib.call("fem_dense_gemm_nb_f32", self.coarse_inv.data_ptr(), b[L - 1].shape[0], b[L - 1].data_ptr(),
@@ -343,8 +358,9 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, x[l + 1].data_ptr(), x[l].data_ptr(), nb, st)
-            _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
-            x[l], t[l] = t[l], x[l]
+            for _ in range(self.sweeps_multi):
+                _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
+                x[l], t[l] = t[l], x[l]
         return x[0]
 
 
