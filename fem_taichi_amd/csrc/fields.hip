@@ -236,43 +236,35 @@ body_force_rows(fem_mesh m, double rho, double w0, double w1, double w2, const d
   }
 }
 
-// Symmetric 3x3 eigenvalues by cyclic Jacobi rotations on the six distinct
-// entries (FP64), descending.  Rotation (p, q): t = tan(phi) is the smaller
-// root of t^2 + 2 t theta - 1 = 0, theta = (a_qq - a_pp) / (2 a_pq), written
-// as t = e sgn(d) / (|d| + hypot(d, e)) with d = a_qq - a_pp, e = 2 a_pq (one
-// square root, one division); then c = 1 / sqrt(1 + t^2), s = t c and the
-// full similarity update, which stays orthogonal however t rounds.
-__device__ __forceinline__ void jrot(double& app, double& aqq, double& apq, double& arp, double& arq) {
-  if (apq == 0.0) return;
-  const double d = aqq - app, e = 2.0 * apq;
-  const double den = fabs(d) + sqrt(fma(d, d, e * e));
-  const double t = (d >= 0.0 ? e : -e) / den;
-  const double c = rsqrt(fma(t, t, 1.0)), sn = t * c;
-  const double cc = c * c, ss = sn * sn, cs = c * sn;
-  const double pp = app, qq = aqq, pq = apq;
-  app = fma(cc, pp, fma(ss, qq, -2.0 * cs * pq));
-  aqq = fma(ss, pp, fma(cc, qq, 2.0 * cs * pq));
-  apq = fma(cs, pp - qq, (cc - ss) * pq);
-  const double rp = arp, rq = arq;
-  arp = fma(c, rp, -sn * rq);
-  arq = fma(sn, rp, c * rq);
-}
-
+// Symmetric 3x3 eigenvalues, descending, in closed form (the trigonometric
+// solution of the characteristic cubic): with q = tr A / 3, p^2 = |A - qI|^2
+// / 6 and B = (A - qI) / p, the eigenvalues are q + 2p cos(phi + 2k pi / 3),
+// phi = acos(det B / 2) / 3 in [0, pi/3], k = 0 (largest), 2 (smallest), the
+// middle one from the trace.  Near a double root acos loses accuracy in phi,
+// but the eigenvalues depend on phi through cos, which is flat there, so
+// their error stays of order eps |A|.
 __device__ void sym3_eigvals_desc(double a00, double a11, double a22, double a01, double a12, double a02,
                                   double ev[3]) {
-  for (int sweep = 0; sweep < 32; ++sweep) {
-    const double off = a01 * a01 + a02 * a02 + a12 * a12;
-    const double dg = a00 * a00 + a11 * a11 + a22 * a22;
-    if (!(off > 1e-36 * dg)) break;   // also exits on NaN
-    jrot(a00, a11, a01, a02, a12);    // (p, q) = (0, 1), r = 2
-    jrot(a00, a22, a02, a01, a12);    // (0, 2), r = 1: a_r0 = a01, a_r2 = a12
-    jrot(a11, a22, a12, a01, a02);    // (1, 2), r = 0: a_r1 = a01, a_r2 = a02
+  const double q = (a00 + a11 + a22) * (1.0 / 3.0);
+  const double b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+  const double off = fma(a01, a01, fma(a02, a02, a12 * a12));
+  const double p2 = fma(b00, b00, fma(b11, b11, fma(b22, b22, 2.0 * off)));
+  if (!(p2 > 0.0)) {                     // A = qI (or NaN, which propagates through q)
+    ev[0] = ev[1] = ev[2] = q + p2;
+    return;
   }
-  double v0 = a00, v1 = a11, v2 = a22, t;
-  if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
-  if (v1 < v2) { t = v1; v1 = v2; v2 = t; }
-  if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
-  ev[0] = v0; ev[1] = v1; ev[2] = v2;
+  const double p = sqrt(p2 * (1.0 / 6.0));
+  const double rp = 1.0 / p;
+  const double c00 = b00 * rp, c11 = b11 * rp, c22 = b22 * rp, c01 = a01 * rp, c12 = a12 * rp, c02 = a02 * rp;
+  const double detb = c00 * fma(c11, c22, -c12 * c12) - c01 * fma(c01, c22, -c12 * c02) +
+                      c02 * fma(c01, c12, -c11 * c02);
+  const double r = fmin(1.0, fmax(-1.0, 0.5 * detb));
+  const double phi = acos(r) * (1.0 / 3.0);
+  const double e0 = fma(2.0 * p, cos(phi), q);
+  const double e2 = fma(2.0 * p, cos(phi + 2.0943951023931957), q);   // + 2 pi / 3
+  ev[0] = e0;
+  ev[1] = 3.0 * q - e0 - e2;
+  ev[2] = e2;
 }
 
 // ---------------------------------------------------------------------------
@@ -410,26 +402,21 @@ stress_rows(fem_mesh m, const double* __restrict__ young, double young_scalar, d
 #pragma unroll
       for (int c = 0; c < 6; ++c) sv[c] += sl[c * 8];
     }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sv[c] /= ce;
+    // principal stresses (descending) and the reference's von Mises term
+    double ev[3];
+    sym3_eigvals_desc(sv[0], sv[1], sv[2], sv[3], sv[4], sv[5], ev);
+    const double vm = sqrt((ev[0] - ev[1]) * (ev[0] - ev[1]) + (ev[1] - ev[2]) * (ev[1] - ev[2]) +
+                           (ev[2] - ev[1]) * (ev[2] - ev[1]));
     double* o = out + i * 10;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) o[c] = sv[c] / ce;
+    for (int c = 0; c < 6; ++c) o[c] = sv[c];
+    o[6] = ev[0];
+    o[7] = ev[1];
+    o[8] = ev[2];
+    o[9] = vm;
   }
-}
-
-// Principal stresses (descending) and the reference's von Mises term, one
-// thread per node, from the averaged stress columns 0..5 of out.
-__global__ void stress_principal(int64_t n, double* __restrict__ out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double* o = out + i * 10;
-  double ev[3];
-  sym3_eigvals_desc(o[0], o[1], o[2], o[3], o[4], o[5], ev);
-  const double vm = sqrt((ev[0] - ev[1]) * (ev[0] - ev[1]) + (ev[1] - ev[2]) * (ev[1] - ev[2]) +
-                         (ev[2] - ev[1]) * (ev[2] - ev[1]));
-  o[6] = ev[0];
-  o[7] = ev[1];
-  o[8] = ev[2];
-  o[9] = vm;
 }
 
 template <int NC>
@@ -496,9 +483,6 @@ extern "C" int fem_stress_recover(const fem_mesh* mesh, const double* young, dou
                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(stress_rows, dim3((unsigned)mesh->num_groups), dim3(kFT), (size_t)lds, st, *mesh, young,
                      young_scalar, lhat, mhat, u, out);
-  FEM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(stress_principal, dim3((unsigned)ceil_div(mesh->num_node, 256)), dim3(256), 0, st,
-                     mesh->num_node, out);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
