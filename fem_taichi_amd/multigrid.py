@@ -242,11 +242,13 @@ class MultigridPreconditioner:
         self.applications = 0
         # damped block-Jacobi sweeps before and after the coarse correction:
         # one for a single vector (the static solve: two take fewer
-        # iterations, no less time), two for the 16-column inner solves of
-        # block Lanczos (30% fewer iterations and 14% less time than one on
-        # the 500 x 60 x 75 blade; three or four take no less time)
+        # iterations, no less time); for the 16-column inner solves of block
+        # Lanczos two on the finest level and three on the coarser ones
+        # (sweeps_multi[min(level, 1)]), 21% less time than one everywhere on
+        # the 500 x 60 x 75 blade (more sweeps, or a Chebyshev sequence of
+        # damping factors, take no less)
         self.sweeps = 1
-        self.sweeps_multi = 2
+        self.sweeps_multi = (2, 3)
         self._multi = {}
         torch.cuda.synchronize()
         self.setup_seconds = time.perf_counter() - t0
@@ -330,6 +332,10 @@ class MultigridPreconditioner:
         self.applications += 1
         return self.vcycle(r.float().contiguous()).double()
 
+    def _sweeps_at(self, level):
+        sw = self.sweeps_multi
+        return sw[min(level, len(sw) - 1)] if isinstance(sw, (list, tuple)) else sw
+
     def vcycle_multi(self, B0: torch.Tensor) -> torch.Tensor:
         """The V-cycle on nb columns at once: B0 (3N, nb) FP32, row-major
         (node-interleaved, columns fastest); returns X (3N, nb) FP32."""
@@ -347,7 +353,7 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_smooth_multi", dev.ref, None, dv, w, None, b[l].data_ptr(), x[l].data_ptr(), nb, 2, st)
-            for _ in range(self.sweeps_multi - 1):
+            for _ in range(self._sweeps_at(l) - 1):
                 _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
                 x[l], t[l] = t[l], x[l]
             _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 0, st)
@@ -358,7 +364,7 @@ class MultigridPreconditioner:
             dev = self.meshes[l]
             k, dv, w = self.k32[l].data_ptr(), self.dinv[l].data_ptr(), self.omega[l]
             _lib.call("fem_mg_prolong_add", self.hier.transfers[l].ref, x[l + 1].data_ptr(), x[l].data_ptr(), nb, st)
-            for _ in range(self.sweeps_multi):
+            for _ in range(self._sweeps_at(l)):
                 _smooth_multi(dev, k, dv, w, x[l], b[l], t[l], nb, 1, st)
                 x[l], t[l] = t[l], x[l]
         return x[0]
