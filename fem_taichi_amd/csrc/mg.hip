@@ -117,16 +117,49 @@ restrict_kernel(fem_grid_transfer t, const float* __restrict__ fine, float* __re
   const int32_t* sz = sy + 2 * t.nc[1];
   const int64_t fx0 = t.nf[0], fxy = (int64_t)t.nf[0] * t.nf[1];
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
-  for (int fz = sz[2 * cz]; fz <= sz[2 * cz + 1]; ++fz) {
-    const float gz = axis_weight(lz, wz, fz, cz);
-    for (int fy = sy[2 * cy]; fy <= sy[2 * cy + 1]; ++fy) {
-      const float gyz = gz * axis_weight(ly, wy, fy, cy);
-      for (int fx = sx[2 * cx]; fx <= sx[2 * cx + 1]; ++fx) {
-        const float g = gyz * axis_weight(lx, wx, fx, cx);
-        const int64_t f = 3 * (fx + fx0 * fy + fxy * fz) * nb + col;
-        a0 += g * fine[f];
-        a1 += g * fine[f + nb];
-        a2 += g * fine[f + 2 * nb];
+  const int z0 = sz[2 * cz], z1 = sz[2 * cz + 1], y0 = sy[2 * cy], y1 = sy[2 * cy + 1];
+  const int x0 = sx[2 * cx], x1 = sx[2 * cx + 1];
+  if (z1 - z0 <= 2 && y1 - y0 <= 2 && x1 - x0 <= 2) {
+    // spans of at most three nodes (every coarsening by axis_select): all
+    // 27 taps unrolled, those past a span's end clamped to it with weight 0,
+    // so the loads issue together; the sums keep the loop order below
+    float gx[3], gy[3], gz[3];
+    int ix[3], iy[3], iz[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      iz[d] = min(z0 + d, z1);
+      iy[d] = min(y0 + d, y1);
+      ix[d] = min(x0 + d, x1);
+      gz[d] = z0 + d <= z1 ? axis_weight(lz, wz, iz[d], cz) : 0.0f;
+      gy[d] = y0 + d <= y1 ? axis_weight(ly, wy, iy[d], cy) : 0.0f;
+      gx[d] = x0 + d <= x1 ? axis_weight(lx, wx, ix[d], cx) : 0.0f;
+    }
+#pragma unroll
+    for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy) {
+        const float gyz = gz[dz] * gy[dy];
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+          const float g = gyz * gx[dx];
+          const int64_t f = 3 * (ix[dx] + fx0 * iy[dy] + fxy * iz[dz]) * nb + col;
+          a0 += g * fine[f];
+          a1 += g * fine[f + nb];
+          a2 += g * fine[f + 2 * nb];
+        }
+      }
+  } else {
+    for (int fz = z0; fz <= z1; ++fz) {
+      const float gz = axis_weight(lz, wz, fz, cz);
+      for (int fy = y0; fy <= y1; ++fy) {
+        const float gyz = gz * axis_weight(ly, wy, fy, cy);
+        for (int fx = x0; fx <= x1; ++fx) {
+          const float g = gyz * axis_weight(lx, wx, fx, cx);
+          const int64_t f = 3 * (fx + fx0 * fy + fxy * fz) * nb + col;
+          a0 += g * fine[f];
+          a1 += g * fine[f + nb];
+          a2 += g * fine[f + 2 * nb];
+        }
       }
     }
   }
@@ -153,24 +186,24 @@ prolong_add_kernel(fem_grid_transfer t, const float* __restrict__ coarse, float*
   const int cx[2] = {lx[fx], lx[fx] + 1}, cy[2] = {ly[fy], ly[fy] + 1}, cz[2] = {lz[fz], lz[fz] + 1};
   const float gx[2] = {1.0f - wx[fx], wx[fx]}, gy[2] = {1.0f - wy[fy], wy[fy]}, gz[2] = {1.0f - wz[fz], wz[fz]};
   const int64_t ncx = t.nc[0], ncxy = (int64_t)t.nc[0] * t.nc[1];
+  // the right neighbour of a coinciding node has weight 0 and may lie past
+  // the grid: its index is clamped and it adds zero, so all eight taps load
+  // together without branches
+  const int cxc[2] = {cx[0], min(cx[1], t.nc[0] - 1)}, cyc[2] = {cy[0], min(cy[1], t.nc[1] - 1)};
+  const int czc[2] = {cz[0], min(cz[1], t.nc[2] - 1)};
   float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
 #pragma unroll
-  for (int kz = 0; kz < 2; ++kz) {
-    if (gz[kz] == 0.0f) continue;      // the right neighbour of a coinciding node (may be out of range)
+  for (int kz = 0; kz < 2; ++kz)
 #pragma unroll
-    for (int ky = 0; ky < 2; ++ky) {
-      if (gy[ky] == 0.0f) continue;
+    for (int ky = 0; ky < 2; ++ky)
 #pragma unroll
       for (int kx = 0; kx < 2; ++kx) {
-        if (gx[kx] == 0.0f) continue;
         const float g = gx[kx] * gy[ky] * gz[kz];
-        const int64_t I = 3 * (cx[kx] + ncx * cy[ky] + ncxy * cz[kz]) * nb + col;
+        const int64_t I = 3 * (cxc[kx] + ncx * cyc[ky] + ncxy * czc[kz]) * nb + col;
         a0 += g * coarse[I];
         a1 += g * coarse[I + nb];
         a2 += g * coarse[I + 2 * nb];
       }
-    }
-  }
   const int64_t o = 3 * f * nb + col;
   fine[o] += a0;
   fine[o + nb] += a1;
