@@ -86,6 +86,13 @@ __device__ __forceinline__ void st_opaque(int& v) { asm volatile("" : "+v"(v)); 
 __device__ __forceinline__ void st_pin(double& v) { asm volatile("" : "+v"(v)); }
 __device__ __forceinline__ void st_pin(float& v) { asm volatile("" : "+v"(v)); }
 
+// Output stores: the 8 lanes of a row write whole 128-byte runs, so streaming
+// (non-temporal) stores lose nothing and keep the outputs from evicting the
+// X halo lines that neighbouring tiles still read (FP64 product 1.38 -> 1.35
+// ms on the 500 x 60 x 75 blade).
+template <typename T>
+__device__ __forceinline__ void st_out(T* p, T v) { __builtin_nontemporal_store(v, p); }
+
 // Blocks of the nodes before index n on an axis of N nodes (a node has
 // 1 + (n > 0) + (n < N - 1) neighbours along the axis, itself included).
 __device__ __forceinline__ int st_prefix(int n, int N) { return n == 0 ? 0 : (N > 1 ? 2 : 1) + 3 * (n - 1); }
@@ -322,8 +329,8 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
       if (MODE == 3) {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
-          OUT[o + p * kNb] = acc[p][0];
-          OUT[o + p * kNb + 1] = acc[p][1];
+          st_out(OUT + o + p * kNb, acc[p][0]);
+          st_out(OUT + o + p * kNb + 1, acc[p][1]);
         }
       } else {
         T res[3][2];
@@ -335,8 +342,8 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
         if (MODE == 0) {
 #pragma unroll
           for (int p = 0; p < 3; ++p) {
-            OUT[o + p * kNb] = res[p][0];
-            OUT[o + p * kNb + 1] = res[p][1];
+            st_out(OUT + o + p * kNb, res[p][0]);
+            st_out(OUT + o + p * kNb + 1, res[p][1]);
           }
         } else {
           // this node's own X row, from the ring
@@ -346,7 +353,7 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
               const T d = w * (dv[3 * p] * res[0][h] + dv[3 * p + 1] * res[1][h] + dv[3 * p + 2] * res[2][h]);
-              OUT[o + p * kNb + h] = xo[p * kNb + h] + d;
+              st_out(OUT + o + p * kNb + h, xo[p * kNb + h] + d);
             }
         }
       }
