@@ -236,13 +236,16 @@ body_force_rows(fem_mesh m, double rho, double w0, double w1, double w2, const d
   }
 }
 
-// Symmetric 3x3 eigenvalues, descending, in closed form (the trigonometric
-// solution of the characteristic cubic): with q = tr A / 3, p^2 = |A - qI|^2
-// / 6 and B = (A - qI) / p, the eigenvalues are q + 2p cos(phi + 2k pi / 3),
-// phi = acos(det B / 2) / 3 in [0, pi/3], k = 0 (largest), 2 (smallest), the
-// middle one from the trace.  Near a double root acos loses accuracy in phi,
-// but the eigenvalues depend on phi through cos, which is flat there, so
-// their error stays of order eps |A|.
+// Symmetric 3x3 eigenvalues, descending.  The trigonometric solution of the
+// characteristic cubic, q + 2p cos(phi + 2k pi / 3) with q = tr A / 3, p^2 =
+// |A - qI|^2 / 6 and phi = acos(det((A - qI) / p) / 2) / 3 in [0, pi/3],
+// gives the root that lies alone at one end accurately (k = 0 when det >= 0,
+// k = 2 otherwise: cos is flat at that root's angle), but the other two only
+// to about sqrt(eps) |A| when they are close (a double root).  So only that
+// root is taken from it: its eigenvector comes from the largest cross
+// product of two rows of A - lambda I, A is projected onto the orthogonal
+// complement, and the 2x2 projection's roots come from its entries, (m00 +
+// m11)/2 +- hypot((m00 - m11)/2, m01), accurate to eps |A| however close.
 __device__ void sym3_eigvals_desc(double a00, double a11, double a22, double a01, double a12, double a02,
                                   double ev[3]) {
   const double q = (a00 + a11 + a22) * (1.0 / 3.0);
@@ -260,11 +263,58 @@ __device__ void sym3_eigvals_desc(double a00, double a11, double a22, double a01
                       c02 * fma(c01, c12, -c11 * c02);
   const double r = fmin(1.0, fmax(-1.0, 0.5 * detb));
   const double phi = acos(r) * (1.0 / 3.0);
-  const double e0 = fma(2.0 * p, cos(phi), q);
-  const double e2 = fma(2.0 * p, cos(phi + 2.0943951023931957), q);   // + 2 pi / 3
-  ev[0] = e0;
-  ev[1] = 3.0 * q - e0 - e2;
-  ev[2] = e2;
+  const bool top = r >= 0.0;             // the lone root is the largest (else the smallest)
+  const double es = fma(2.0 * p, top ? cos(phi) : cos(phi + 2.0943951023931957), q);   // + 2 pi / 3
+  // its eigenvector: the largest cross product of two rows of A - es I
+  const double r0x = a00 - es, r1y = a11 - es, r2z = a22 - es;
+  double vx = a01 * a12 - a02 * r1y, vy = a02 * a01 - r0x * a12, vz = r0x * r1y - a01 * a01;   // row0 x row1
+  double nv = fma(vx, vx, fma(vy, vy, vz * vz));
+  {
+    const double x = a01 * r2z - a02 * a12, y = a02 * a02 - r0x * r2z, z = r0x * a12 - a01 * a02;  // row0 x row2
+    const double n = fma(x, x, fma(y, y, z * z));
+    if (n > nv) { vx = x; vy = y; vz = z; nv = n; }
+  }
+  {
+    const double x = r1y * r2z - a12 * a12, y = a12 * a02 - a01 * r2z, z = a01 * a12 - r1y * a02;  // row1 x row2
+    const double n = fma(x, x, fma(y, y, z * z));
+    if (n > nv) { vx = x; vy = y; vz = z; nv = n; }
+  }
+  double e1, e2;
+  if (!(nv > 0.0)) {                     // no usable direction: keep the cubic's other roots
+    const double eo = fma(2.0 * p, top ? cos(phi + 2.0943951023931957) : cos(phi), q);
+    e1 = fmax(eo, 3.0 * q - es - eo);
+    e2 = fmin(eo, 3.0 * q - es - eo);
+  } else {
+    const double rn = 1.0 / sqrt(nv);
+    vx *= rn; vy *= rn; vz *= rn;
+    // orthonormal u, w spanning the complement of v
+    double ux, uy, uz;
+    if (fabs(vx) > fabs(vy)) {
+      const double ru = 1.0 / sqrt(fma(vx, vx, vz * vz));
+      ux = -vz * ru; uy = 0.0; uz = vx * ru;
+    } else {
+      const double ru = 1.0 / sqrt(fma(vy, vy, vz * vz));
+      ux = 0.0; uy = vz * ru; uz = -vy * ru;
+    }
+    const double wx = vy * uz - vz * uy, wy = vz * ux - vx * uz, wz = vx * uy - vy * ux;
+    const double aux = fma(a00, ux, fma(a01, uy, a02 * uz)), auy = fma(a01, ux, fma(a11, uy, a12 * uz)),
+                 auz = fma(a02, ux, fma(a12, uy, a22 * uz));
+    const double awx = fma(a00, wx, fma(a01, wy, a02 * wz)), awy = fma(a01, wx, fma(a11, wy, a12 * wz)),
+                 awz = fma(a02, wx, fma(a12, wy, a22 * wz));
+    const double m00 = fma(ux, aux, fma(uy, auy, uz * auz));
+    const double m11 = fma(wx, awx, fma(wy, awy, wz * awz));
+    const double m01 = fma(ux, awx, fma(uy, awy, uz * awz));
+    const double h = 0.5 * (m00 + m11), d = hypot(0.5 * (m00 - m11), m01);
+    e1 = h + d;
+    e2 = h - d;
+  }
+  double v0 = top ? es : e1, v1 = top ? e1 : e2, v2 = top ? e2 : es, t;
+  if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
+  if (v1 < v2) { t = v1; v1 = v2; v2 = t; }
+  if (v0 < v1) { t = v0; v0 = v1; v1 = t; }
+  ev[0] = v0;
+  ev[1] = v1;
+  ev[2] = v2;
 }
 
 // ---------------------------------------------------------------------------

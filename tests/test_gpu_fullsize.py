@@ -116,6 +116,41 @@ def test_affine_displacement_stress(slab):
     assert np.linalg.norm(S[:, 6:9] - ebar[:, None] * ev[None, :]) / np.linalg.norm(ebar[:, None] * ev) < 1e-10
 
 
+@pytest.mark.parametrize("case", ["hydrostatic", "uniaxial", "shear", "rotation"])
+def test_affine_stress_degenerate_principal(slab, case):
+    """Principal stresses and the reference's von Mises term where the
+    closed-form 3x3 solve meets repeated roots: a triple root (hydrostatic
+    strain), a double root (uniaxial strain), roots (t, 0, -t) (pure shear)
+    and a zero tensor (a rigid rotation, whose stresses are rounding
+    noise)."""
+    from fem_taichi_amd import Post
+    g, phys, model = slab
+    xyz = np.asarray(g.Node[-3:], dtype=np.float64).T
+    G = {"hydrostatic": np.eye(3), "uniaxial": np.diag([1.0, 0.0, 0.0]),
+         "shear": np.array([[0.0, 1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 0.0]]),
+         "rotation": np.array([[0.0, -1.0, 0.5], [1.0, 0.0, -0.3], [-0.5, 0.3, 0.0]])}[case] * 1e-4
+    S = Post(g.name, phys, g.Materials, (xyz @ G.T).reshape(-1, 3, 1)).Get_Stress()
+    nu = float(g.Materials["PO"])
+    lhat, mhat = nu / ((1 + nu) * (1 - 2 * nu)), 1.0 / (2 * (1 + nu))
+    eps = 0.5 * (G + G.T)
+    s_unit = lhat * np.trace(eps) * np.eye(3) + 2 * mhat * eps
+    en = np.asarray(g.Elem[-8:], dtype=np.int64).T
+    E = np.asarray(g.Materials["EX"], dtype=np.float64)
+    N = xyz.shape[0]
+    ebar = np.bincount(en.reshape(-1), weights=np.repeat(E, 8), minlength=N) / np.bincount(en.reshape(-1), minlength=N)
+    if case == "rotation":
+        assert np.max(np.abs(S)) < 1e-12 * E.max() * 1e-4
+        return
+    ev = np.sort(np.linalg.eigvalsh(s_unit))[::-1]
+    expect = ebar[:, None] * ev[None, :]
+    assert np.linalg.norm(S[:, 6:9] - expect) / np.linalg.norm(expect) < 1e-10
+    vm = np.sqrt((ev[0] - ev[1]) ** 2 + (ev[1] - ev[2]) ** 2 + (ev[2] - ev[1]) ** 2)   # as written in the reference
+    if vm > 0:
+        assert np.linalg.norm(S[:, 9] - ebar * vm) / np.linalg.norm(ebar * vm) < 1e-8
+    else:
+        assert np.max(np.abs(S[:, 9])) < 1e-6 * np.max(np.abs(expect))
+
+
 @pytest.fixture(scope="module")
 def blade(gpu):
     """The static-solve and modal workload: twisted blade 400x50x50 (1.04M
