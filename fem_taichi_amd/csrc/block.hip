@@ -809,12 +809,27 @@ gemm_nn_kernel(const double* __restrict__ A, int64_t lda, const double* __restri
       for (int c = 0; c < QMAX; ++c) acc[c] = fma(a, sC[j * QMAX + c], acc[c]);
     }
   }
-  const int64_t row = row0 + tid;
-  if (row >= n) return;
-  double* y = Y + row * ldy;
+  // Y through LDS, 16 columns at a time, so that consecutive lanes read and
+  // write consecutive addresses (a thread's own row would make every store
+  // instruction touch 64 lines for 8 bytes each)
 #pragma unroll
-  for (int c = 0; c < QMAX; ++c) {
-    if (c < q) y[c] = beta == 0.0 ? alpha * acc[c] : fma(alpha, acc[c], beta * y[c]);
+  for (int c0 = 0; c0 < QMAX; c0 += kNnChunk) {
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < kNnChunk && cc < QMAX; ++cc) sA[tid * (kNnChunk + 1) + cc] = acc[c0 + cc];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kNnChunk; ++i) {
+      const int e = tid + i * kNnT;
+      const int r = e / kNnChunk, cc = e - r * kNnChunk;
+      const int64_t row = row0 + r;
+      const int c = c0 + cc;
+      if (row < n && c < q && cc < QMAX) {
+        const double a = sA[r * (kNnChunk + 1) + cc];
+        double* y = Y + row * ldy + c;
+        *y = beta == 0.0 ? alpha * a : fma(alpha, a, beta * *y);
+      }
+    }
   }
 }
 
