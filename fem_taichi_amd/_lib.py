@@ -95,6 +95,8 @@ SIGNATURES = {
     "fem_bsr_spmm": (_I32, [_MP, _P, _P, _P, _I64, _P]),
     "fem_mass_spmm": (_I32, [_MP, _P, _P, _P, _I64, _P]),
     "fem_bsr_spmm16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, _P, _P]),
+    "fem_spmm16_grid_partials": (_I64, [_I64, _I64, _I64]),
+    "fem_bsr_spmm16_grid_pq": (_I32, [_MP, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _P]),
     "fem_mass_spmm16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, _P, _P]),
     "fem_mg_smooth16_grid": (_I32, [_MP, _I64, _I64, _I64, _P, _P, ctypes.c_float, _P, _P, _P, _I32, _P]),
     "fem_bj_setup": (_I32, [_MP, _P, _I32, _P, _P, _P]),
@@ -132,6 +134,8 @@ SIGNATURES = {
     "fem_fcg_direction": (_I32, [_I64, _I64, _P, _I32, _P, _P, _P, _I32, _P, _I64, _P]),
     "fem_fcg_update": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
     "fem_fcg_residual": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _P, _P, _P, _I64, _P]),
+    "fem_fcg_direction_pr": (_I32, [_I64, _I64, _P, _I32, _P, _P, _P, _P, _I32, _P, _I64, _P]),
+    "fem_fcg_update_split": (_I32, [_I64, _I64, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _I64, _P]),
 }
 
 _lib = None

@@ -463,6 +463,22 @@ fcg_alpha_kernel(const double* __restrict__ part_pq, const double* __restrict__ 
   }
 }
 
+// alpha_c = pr / pq from partial sets of different counts (pq from the
+// structured-grid product's work-groups, pr from the direction pass)
+__global__ void __launch_bounds__(kFoldT)
+fcg_alpha_split_kernel(const double* __restrict__ part_pq, int nblk_pq, const double* __restrict__ part_pr,
+                       int nblk_pr, int nb, const int32_t* __restrict__ active, double* __restrict__ sc) {
+  __shared__ double red[kFoldT];
+  int c;
+  const double pq = fcg_fold_partials(part_pq, nblk_pq, nb, red, &c);
+  const double pr = fcg_fold_partials(part_pr, nblk_pr, nb, red, &c);
+  if (c < nb) {
+    const double pq_safe = pq != 0.0 ? pq : 1.0;
+    sc[kScAlpha * nb + c] = (active == nullptr || active[c]) ? pr / pq_safe : 0.0;
+    sc[kScPqPrev * nb + c] = pq_safe;
+  }
+}
+
 __global__ void __launch_bounds__(kFoldT)
 fcg_sum_kernel(const double* __restrict__ part, int nblk, int nb, double* __restrict__ out) {
   __shared__ double red[kFoldT];
@@ -471,20 +487,30 @@ fcg_sum_kernel(const double* __restrict__ part, int nblk, int nb, double* __rest
   if (c < nb) out[c] = v;
 }
 
-// P = mask * Z - beta * P   (first: P = mask * Z)
-template <typename TZ>
+// P = mask * Z - beta * P   (first: P = mask * Z); PR: also the partials
+// of sum P * R (the numerator of alpha) of the new P
+template <typename TZ, bool PR>
 __global__ void __launch_bounds__(kFT)
 fcg_dir_kernel(int64_t n, int nb, const TZ* __restrict__ Z, const double* __restrict__ mask,
-               const double* __restrict__ sc, int first, double* __restrict__ P) {
+               const double* __restrict__ sc, int first, double* __restrict__ P, const double* __restrict__ R,
+               double* __restrict__ part_pr) {
   const FcgMap f = fcg_map(n, nb);
-  if (!f.on) return;
-  const double beta = first ? 0.0 : sc[kScBeta * nb + f.c];
+  double pr = 0.0;
+  if (f.on) {
+    const double beta = first ? 0.0 : sc[kScBeta * nb + f.c];
 #pragma unroll 4
-  for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
-    const int64_t e = i * nb + f.c;
-    double z = (double)Z[e];
-    if (mask) z *= mask[i];
-    P[e] = first ? z : z - beta * P[e];
+    for (int64_t i = f.r0 + f.s; i < f.r1; i += f.G) {
+      const int64_t e = i * nb + f.c;
+      double z = (double)Z[e];
+      if (mask) z *= mask[i];
+      const double p = first ? z : z - beta * P[e];
+      P[e] = p;
+      if (PR) pr += p * R[e];
+    }
+  }
+  if (PR) {
+    __shared__ double red[kFT];
+    fcg_block_fold(pr, f, nb, red, part_pr + (int64_t)blockIdx.x * nb);
   }
 }
 
@@ -704,8 +730,8 @@ extern "C" int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dty
     fcg_beta_kernel<<<1, kFoldT, 0, st>>>(part, grid, nbi, sc);
     FEM_LAUNCH_CHECK();
   }
-  if (f32) fcg_dir_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, mask, sc, first ? 1 : 0, P);
-  else fcg_dir_kernel<double><<<grid, kFT, 0, st>>>(n, nbi, Zd, mask, sc, first ? 1 : 0, P);
+  if (f32) fcg_dir_kernel<float, false><<<grid, kFT, 0, st>>>(n, nbi, Zf, mask, sc, first ? 1 : 0, P, nullptr, nullptr);
+  else fcg_dir_kernel<double, false><<<grid, kFT, 0, st>>>(n, nbi, Zd, mask, sc, first ? 1 : 0, P, nullptr, nullptr);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -727,6 +753,61 @@ extern "C" int fem_fcg_update(int64_t n, int64_t nb, const double* P, const doub
   fcg_dot2_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, R, part, part2);
   FEM_LAUNCH_CHECK();
   fcg_alpha_kernel<<<1, kFoldT, 0, st>>>(part, part2, grid, nbi, active, sc);
+  FEM_LAUNCH_CHECK();
+  fcg_update_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, mask, sc, X, R, R32, part3);
+  FEM_LAUNCH_CHECK();
+  fcg_sum_kernel<<<1, kFoldT, 0, st>>>(part3, grid, nbi, rr);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_fcg_direction_pr(int64_t n, int64_t nb, const void* Z, int z_dtype, const double* mask,
+                                    const double* Qp, const double* R, double* P, int first, void* work,
+                                    int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64, FEM_E_INVALID, "nb must be in [1, 64] (got %lld)", (long long)nb);
+  FEM_REQUIRE(z_dtype == FEM_DTYPE_F32 || z_dtype == FEM_DTYPE_F64, FEM_E_INVALID, "bad z_dtype %d", z_dtype);
+  FEM_REQUIRE(n >= 0, FEM_E_INVALID, "negative row count");
+  FEM_REQUIRE(work && work_bytes >= fem_fcg_work_bytes(nb), FEM_E_INVALID, "work too small");
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  double* part2 = part + (int64_t)kFcgGrid * nb;     // sum P R, read by fem_fcg_update_split
+  double* sc = part + 3 * (int64_t)kFcgGrid * nb;
+  const int grid = fcg_grid(n), nbi = (int)nb;
+  if (n > 0) FEM_REQUIRE(Z && P && R && (first || Qp), FEM_E_INVALID, "null pointer");
+  const bool f32 = z_dtype == FEM_DTYPE_F32;
+  const float* Zf = reinterpret_cast<const float*>(Z);
+  const double* Zd = reinterpret_cast<const double*>(Z);
+  if (!first && n > 0) {
+    if (f32) fcg_dot_kernel<float><<<grid, kFT, 0, st>>>(n, nbi, Zf, Qp, mask, part);
+    else fcg_dot_kernel<double><<<grid, kFT, 0, st>>>(n, nbi, Zd, Qp, mask, part);
+    FEM_LAUNCH_CHECK();
+    fcg_beta_kernel<<<1, kFoldT, 0, st>>>(part, grid, nbi, sc);
+    FEM_LAUNCH_CHECK();
+  }
+  // with no rows the one work-group only writes zero partials
+  const int fst = first || n == 0 ? 1 : 0;
+  if (f32) fcg_dir_kernel<float, true><<<grid, kFT, 0, st>>>(n, nbi, Zf, mask, sc, fst, P, R, part2);
+  else fcg_dir_kernel<double, true><<<grid, kFT, 0, st>>>(n, nbi, Zd, mask, sc, fst, P, R, part2);
+  FEM_LAUNCH_CHECK();
+  return FEM_OK;
+}
+
+extern "C" int fem_fcg_update_split(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
+                                    const int32_t* active, const double* pq_part, int64_t pq_blocks, double* X,
+                                    double* R, float* R32, double* rr, void* work, int64_t work_bytes, void* stream) {
+  FEM_REQUIRE(nb >= 1 && nb <= 64, FEM_E_INVALID, "nb must be in [1, 64] (got %lld)", (long long)nb);
+  FEM_REQUIRE(n >= 0, FEM_E_INVALID, "negative row count");
+  FEM_REQUIRE(work && work_bytes >= fem_fcg_work_bytes(nb), FEM_E_INVALID, "work too small");
+  FEM_REQUIRE(rr && pq_part, FEM_E_INVALID, "null rr or pq_part");
+  FEM_REQUIRE(pq_blocks >= 1 && pq_blocks < INT32_MAX, FEM_E_INVALID, "bad pq_blocks %lld", (long long)pq_blocks);
+  hipStream_t st = as_stream(stream);
+  double* part = reinterpret_cast<double*>(work);
+  double* part2 = part + (int64_t)kFcgGrid * nb;
+  double* part3 = part2 + (int64_t)kFcgGrid * nb;
+  double* sc = part + 3 * (int64_t)kFcgGrid * nb;
+  const int grid = fcg_grid(n), nbi = (int)nb;
+  if (n > 0) FEM_REQUIRE(P && Q && X && R, FEM_E_INVALID, "null pointer");
+  fcg_alpha_split_kernel<<<1, kFoldT, 0, st>>>(pq_part, (int)pq_blocks, part2, grid, nbi, active, sc);
   FEM_LAUNCH_CHECK();
   fcg_update_kernel<<<grid, kFT, 0, st>>>(n, nbi, P, Q, mask, sc, X, R, R32, part3);
   FEM_LAUNCH_CHECK();

@@ -97,14 +97,16 @@ __device__ __forceinline__ void st_out(T* p, T v) { __builtin_nontemporal_store(
 // 1 + (n > 0) + (n < N - 1) neighbours along the axis, itself included).
 __device__ __forceinline__ int st_prefix(int n, int N) { return n == 0 ? 0 : (N > 1 ? 2 : 1) + 3 * (n - 1); }
 
-// MODE 0: OUT = B - K X;  1: OUT = X + w Dinv (B - K X);  3: OUT = K X.
+// MODE 0: OUT = B - K X;  1: OUT = X + w Dinv (B - K X);  3: OUT = K X;
+// 4: OUT = K X and part[work item][c] = sum over the item's nodes of
+// X_c . (K X)_c (FP64: the p . q of conjugate gradients, in a fixed order).
 // BW: values per block, 9 (3 x 3 blocks of K) or 1 (the mass matrix's one
 // scalar per node pair on each DOF diagonal).
 template <typename T, int MODE, int BW, int PF>
 __global__ void __launch_bounds__(kSt, sizeof(T) == 8 ? 2 : 3)
 stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* __restrict__ B,
                  const T* __restrict__ dinv, T w, T* __restrict__ OUT, int NX, int NY, int NZ, int seg, int ntx,
-                 int nty, int nwork, int per_xcd) {
+                 int nty, int nwork, int per_xcd, double* __restrict__ part) {
   using G = StGeom<T>;
   using V = typename G::V;
   constexpr int S = G::S, RP = G::RP, PL = G::PL;
@@ -241,6 +243,7 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
 
   if (tid < S / CW) *reinterpret_cast<V*>(zrow + tid * CW) = V{};
   V pre[LPT];
+  double pq0 = 0.0, pq1 = 0.0;   // MODE 4: this lane's columns c0, c0 + 1
   if (z0 > 0) {
     fetch(z0 - 1, pre);
     stash(z0 - 1, pre);
@@ -275,7 +278,7 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
     const int inode = rx + NX * (ry + NY * z);
     const int64_t o = (int64_t)inode * (3 * kNb) + c0;
     T bv[3][2], dv[9];
-    if (MODE != 3 && row_ok) {
+    if (MODE < 3 && row_ok) {
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
         bv[p][0] = B[o + p * kNb];
@@ -326,11 +329,20 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
     }
     // slots 0, 1 now hold columns c0, c0 + 1 of all three components
     if (row_ok) {
-      if (MODE == 3) {
+      if (MODE == 3 || MODE == 4) {
 #pragma unroll
         for (int p = 0; p < 3; ++p) {
           st_out(OUT + o + p * kNb, acc[p][0]);
           st_out(OUT + o + p * kNb + 1, acc[p][1]);
+        }
+        if (MODE == 4) {
+          // this node's own X row, from the ring
+          const T* xo = ring + (z % 3) * PL + (ry - y0 + 1) * RP + (rx - x0 + 1) * S + c0;
+#pragma unroll
+          for (int p = 0; p < 3; ++p) {
+            pq0 = fma((double)xo[p * kNb], (double)acc[p][0], pq0);
+            pq1 = fma((double)xo[p * kNb + 1], (double)acc[p][1], pq1);
+          }
         }
       } else {
         T res[3][2];
@@ -360,6 +372,25 @@ stencil16_kernel(const T* __restrict__ vals, const T* __restrict__ X, const T* _
     }
     st_lds_barrier();   // every wave is done with plane z-1's slot before it takes plane z+2
   }
+  if (MODE == 4) {
+    // column c sits in slot c & 1 of the lane j whose c0 is c & ~1; sum the
+    // 32 rows of the tile in row order (the ring is free after the last barrier)
+    double* red = reinterpret_cast<double*>(smem);
+    red[2 * tid] = pq0;
+    red[2 * tid + 1] = pq1;
+    __syncthreads();
+    if (tid < kNb) {
+      int jc = 0;
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const int a0 = jj & 1, a1 = (jj >> 1) & 1, a2 = jj >> 2;
+        if (8 * a2 + 4 * (a1 ^ a2) + 2 * (a0 ^ a2) == (tid & ~1)) jc = jj;
+      }
+      double sum = 0.0;
+      for (int rr = 0; rr < kSt / 8; ++rr) sum += red[2 * (rr * 8 + jc) + (tid & 1)];
+      part[(int64_t)wid * kNb + tid] = sum;
+    }
+  }
 }
 
 int g_st_cus[64] = {0};
@@ -375,25 +406,40 @@ int st_cu_count() {
   return g_st_cus[dev];
 }
 
+// Launch shape of a grid: 8 x 4 tiles times z segments, enough work-groups
+// for two rounds over every CU's resident ones.
+template <typename T>
+struct StPlan {
+  int ntx, nty, seg;
+  int64_t nwork, per_xcd;
+  StPlan(int64_t nx, int64_t ny, int64_t nz) {
+    constexpr int lds = st_lds_bytes<T>();
+    ntx = (int)ceil_div(nx, kStTX);
+    nty = (int)ceil_div(ny, kStTY);
+    const int64_t tiles = (int64_t)ntx * nty;
+    const int64_t per_cu = std::min<int64_t>(sizeof(T) == 8 ? 2 : 3, (160 * 1024) / lds);
+    const int64_t resident = (int64_t)st_cu_count() * per_cu;
+    const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(nz, ceil_div(2 * resident, tiles)));
+    seg = (int)ceil_div(nz, nseg);
+    nwork = tiles * ceil_div(nz, seg);
+    per_xcd = ceil_div(nwork, kNumXcd);
+  }
+};
+
 template <typename T, int MODE, int BW = 9>
 int launch_stencil16(int64_t nx, int64_t ny, int64_t nz, const T* vals, const T* X, const T* B, const T* dinv, T w,
-                     T* OUT, hipStream_t st) {
+                     T* OUT, hipStream_t st, double* part = nullptr, int64_t part_len = 0) {
   constexpr int lds = st_lds_bytes<T>();
   auto kern = stencil16_kernel<T, MODE, BW, (sizeof(T) == 8 && BW == 9) || MODE == 1 ? 2 : 4>;
   FEM_CHECK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lds));
-  const int ntx = (int)ceil_div(nx, kStTX), nty = (int)ceil_div(ny, kStTY);
-  const int64_t tiles = (int64_t)ntx * nty;
-  // z segments: enough work-groups for two rounds over every CU's resident ones
-  const int64_t per_cu = std::min<int64_t>(sizeof(T) == 8 ? 2 : 3, (160 * 1024) / lds);
-  const int64_t resident = (int64_t)st_cu_count() * per_cu;
-  const int64_t nseg = std::max<int64_t>(1, std::min<int64_t>(nz, ceil_div(2 * resident, tiles)));
-  const int seg = (int)ceil_div(nz, nseg);
-  const int64_t nwork = tiles * ceil_div(nz, seg);
-  const int64_t per_xcd = ceil_div(nwork, kNumXcd);
-  FEM_REQUIRE(per_xcd * kNumXcd < ((int64_t)1 << 31), FEM_E_LIMIT, "grid too large for one launch");
-  hipLaunchKernelGGL(kern, dim3((unsigned)(per_xcd * kNumXcd)), dim3(kSt), lds, st, vals, X, B, dinv, w, OUT,
-                     (int)nx, (int)ny, (int)nz, seg, ntx, nty, (int)nwork, (int)per_xcd);
+  const StPlan<T> pl(nx, ny, nz);
+  FEM_REQUIRE(pl.per_xcd * kNumXcd < ((int64_t)1 << 31), FEM_E_LIMIT, "grid too large for one launch");
+  if (MODE == 4)
+    FEM_REQUIRE(part && part_len >= pl.nwork * kNb, FEM_E_INVALID,
+                "partials need %lld doubles (got %lld)", (long long)(pl.nwork * kNb), (long long)part_len);
+  hipLaunchKernelGGL(kern, dim3((unsigned)(pl.per_xcd * kNumXcd)), dim3(kSt), lds, st, vals, X, B, dinv, w, OUT,
+                     (int)nx, (int)ny, (int)nz, pl.seg, pl.ntx, pl.nty, (int)pl.nwork, (int)pl.per_xcd, part);
   FEM_LAUNCH_CHECK();
   return FEM_OK;
 }
@@ -447,4 +493,19 @@ extern "C" int fem_mg_smooth16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny
   const hipStream_t st = as_stream(stream);
   if (mode == 0) return launch_stencil16<float, 0>(nx, ny, nz, vals, X, B, dinv, omega, OUT, st);
   return launch_stencil16<float, 1>(nx, ny, nz, vals, X, B, dinv, omega, OUT, st);
+}
+
+extern "C" int64_t fem_spmm16_grid_partials(int64_t nx, int64_t ny, int64_t nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return 0;
+  return StPlan<double>(nx, ny, nz).nwork;
+}
+
+extern "C" int fem_bsr_spmm16_grid_pq(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* vals,
+                                      const double* X, double* Y, double* part, int64_t part_len, void* stream) {
+  int rc = check_grid16(mesh, nx, ny, nz);
+  if (rc) return rc;
+  FEM_REQUIRE(vals && X && Y && X != Y, FEM_E_INVALID, "null pointer, or Y aliases X");
+  FEM_REQUIRE(reinterpret_cast<uintptr_t>(X) % 16 == 0, FEM_E_INVALID, "X must be 16-byte aligned");
+  return launch_stencil16<double, 4>(nx, ny, nz, vals, X, nullptr, nullptr, 0.0, Y, as_stream(stream), part,
+                                     part_len);
 }

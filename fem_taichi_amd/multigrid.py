@@ -418,6 +418,28 @@ class FcgSteps:
                   stream_ptr())
         return self.rr
 
+    def direction_pr(self, Z, mask, Qp, R, P, first):
+        """direction, also leaving the partials of p.r (R: the current
+        residual) for update_split."""
+        if Z.dtype not in (torch.float32, torch.float64) or P.dtype != torch.float64:
+            raise ValueError("FCG direction needs an FP32 or FP64 Z and an FP64 P")
+        Z = Z.contiguous()
+        self._check(Z, Qp, R, P)
+        zdt = _lib.DTYPE_F32 if Z.dtype == torch.float32 else _lib.DTYPE_F64
+        _lib.call("fem_fcg_direction_pr", self.n, self.nb, Z.data_ptr(), zdt, _ptr(mask), _ptr(Qp), R.data_ptr(),
+                  P.data_ptr(), 1 if first else 0, self.work.data_ptr(), self.wb, stream_ptr())
+
+    def update_split(self, P, Q, mask, active, pq_part, pq_blocks, X, R, R32):
+        """update with p.q from pq_blocks partials pq_part (nb per block, as
+        fem_bsr_spmm16_grid_pq leaves them) and p.r from direction_pr."""
+        self._check(P, Q, X, R, R32)
+        if pq_part.dtype != torch.float64 or pq_part.numel() < pq_blocks * self.nb:
+            raise ValueError("pq_part must hold pq_blocks x nb doubles")
+        _lib.call("fem_fcg_update_split", self.n, self.nb, P.data_ptr(), Q.data_ptr(), _ptr(mask), _ptr(active),
+                  pq_part.data_ptr(), int(pq_blocks), X.data_ptr(), R.data_ptr(), _ptr(R32), self.rr.data_ptr(),
+                  self.work.data_ptr(), self.wb, stream_ptr())
+        return self.rr
+
     def residual(self, B, Y, mask, sel, R, R32):
         """t = (B - Y) mask (Y None: B mask); R = t and R32 = (float) t on the
         columns with sel[c] != 0 (sel None: all; R, R32 may be None).
@@ -528,8 +550,9 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
                        drop=3e-3, check_after=50, floor=1e-7):
     """solve_fcg_mg for the nb columns of B (3N, nb) at once: per-column
     FCG(1) scalars, one multi-column V-cycle (precond.vcycle_multi) on the
-    FP32 copy of the residual, one FP64 SpMM (fem_bsr_spmm) and the fused
-    vector steps (FcgSteps) per iteration.  Each column stops on its own
+    FP32 copy of the residual, one FP64 SpMM (fem_bsr_spmm, or on a
+    structured grid with nb = 16 fem_bsr_spmm16_grid_pq, which also leaves
+    p.q) and the fused vector steps (FcgSteps) per iteration.  Each column stops on its own
     (tolerance, or the attainable accuracy as in solve_fcg_mg) and is frozen
     from then on.  A column asks for a true-residual check (one product of
     all columns, which replaces its recursive residual) when its recursive
@@ -574,13 +597,27 @@ def solve_block_fcg_mg(model, K64, B, precond, tol=1e-8, maxiter=1000, free=None
     act_d = torch.as_tensor(active.astype(np.int32), device=B.device)
     P = torch.empty_like(B)
     Q, Qp = torch.empty_like(B), torch.empty_like(B)
+    # on a structured grid with 16 columns the product also leaves p.q
+    # (fem_bsr_spmm16_grid_pq), and p.r comes with the direction: no extra
+    # pass over P, Q and R
+    grid = getattr(K64.mesh, "grid_nodes", None)
+    fused = nb == 16 and grid is not None
+    if fused:
+        pq_blocks = int(_lib.lib().fem_spmm16_grid_partials(*grid))
+        pq_part = torch.empty(pq_blocks * nb, dtype=torch.float64, device=B.device)
     first = True
     it = 0
     while it < maxiter and active.any():
         Z = precond.vcycle_multi(R32)
-        steps.direction(Z, free, None if first else Qp, P, first)
-        solvers.bsr_spmm(K64.mesh, K64.vals, P, Q, nb)
-        rr = steps.update(P, Q, free, act_d, X, R, R32)
+        if fused:
+            steps.direction_pr(Z, free, None if first else Qp, R, P, first)
+            _lib.call("fem_bsr_spmm16_grid_pq", K64.mesh.ref, grid[0], grid[1], grid[2], K64.vals.data_ptr(),
+                      P.data_ptr(), Q.data_ptr(), pq_part.data_ptr(), pq_part.numel(), stream_ptr())
+            rr = steps.update_split(P, Q, free, act_d, pq_part, pq_blocks, X, R, R32)
+        else:
+            steps.direction(Z, free, None if first else Qp, P, first)
+            solvers.bsr_spmm(K64.mesh, K64.vals, P, Q, nb)
+            rr = steps.update(P, Q, free, act_d, X, R, R32)
         Q, Qp = Qp, Q
         first = False
         it += 1

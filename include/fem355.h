@@ -234,6 +234,15 @@ int fem_bsr_spmm(const fem_mesh* mesh, const double* vals, const double* X, doub
  * fem_bsr_spmm's, so results agree to rounding.  X 16-byte aligned, Y != X. */
 int fem_bsr_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* vals,
                         const double* X, double* Y, void* stream);
+/* fem_bsr_spmm16_grid that also leaves, per work-group w and column c,
+ * part[w*16 + c] = sum over the work-group's rows i of X_ic Y_ic (the p.q of
+ * flexible CG read off the product; no counterpart in the reference, whose
+ * solves are direct).  part holds part_len doubles, at least
+ * 16 * fem_spmm16_grid_partials(nx, ny, nz); the count is per device (it
+ * follows the CU count), so query it on the device that runs the product. */
+int64_t fem_spmm16_grid_partials(int64_t nx, int64_t ny, int64_t nz);
+int fem_bsr_spmm16_grid_pq(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* vals,
+                           const double* X, double* Y, double* part, int64_t part_len, void* stream);
 /* fem_mass_spmm with 16 columns on a structured node grid (the layout and
  * conditions of fem_bsr_spmm16_grid). */
 int fem_mass_spmm16_grid(const fem_mesh* mesh, int64_t nx, int64_t ny, int64_t nz, const double* m_blocks,
@@ -398,6 +407,20 @@ int fem_fcg_direction(int64_t n, int64_t nb, const void* Z, int z_dtype, const d
 int fem_fcg_update(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
                    const int32_t* active, double* X, double* R, float* R32, double* rr, void* work,
                    int64_t work_bytes, void* stream);
+/* The same two steps with p.q taken from fem_bsr_spmm16_grid_pq's partials
+ * instead of a pass over P and Q (no counterpart in the reference):
+ *   direction_pr: fem_fcg_direction that also leaves the partials of p.r of
+ *                 the new P (R: the current residual) in work;
+ *   update_split: alpha = p.r / p.q from those and the pq_blocks partials of
+ *                 pq_part (pq_part[w*nb + c]), then fem_fcg_update's X, R,
+ *                 R32 and rr.
+ * Between the two, run the product Q = K P that fills pq_part. */
+int fem_fcg_direction_pr(int64_t n, int64_t nb, const void* Z, int z_dtype, const double* mask,
+                         const double* Qp, const double* R, double* P, int first, void* work, int64_t work_bytes,
+                         void* stream);
+int fem_fcg_update_split(int64_t n, int64_t nb, const double* P, const double* Q, const double* mask,
+                         const int32_t* active, const double* pq_part, int64_t pq_blocks, double* X, double* R,
+                         float* R32, double* rr, void* work, int64_t work_bytes, void* stream);
 /* The residual of the block solve (multigrid.py solve_block_fcg_mg's start,
  * true-residual checks and final norms, in one pass; no counterpart kernel in
  * the reference): t = (B - Y) * mask (Y NULL: B * mask; mask NULL: all

@@ -129,9 +129,12 @@ def test_smoother_multi_matches_single(gpu, nb):
                 assert torch.equal(Oh, OUT[:, 8 * h:8 * h + 8]), (mode, h)
 
 
-def test_block_fcg_matches_scipy(gpu):
-    """The block flexible CG (8 right-hand sides, one of them zero) against
-    scipy's direct solves on the oracle's K: every nonzero column to 1e-8."""
+@pytest.mark.parametrize("nb", [8, 16])
+def test_block_fcg_matches_scipy(gpu, nb):
+    """The block flexible CG (8 or 16 right-hand sides, one of them zero)
+    against scipy's direct solves on the oracle's K: every nonzero column to
+    1e-9.  With 16 columns on the blade's node grid the iteration takes p.q
+    from the structured-grid product (fem_bsr_spmm16_grid_pq)."""
     import torch
     import scipy.sparse.linalg as sla
     from fem_taichi_amd import Model, Physics, solvers, multigrid as MG
@@ -142,7 +145,8 @@ def test_block_fcg_matches_scipy(gpu):
     pre = MG.MultigridPreconditioner(model, MG.hierarchy_for(model))
     pp = O.pre_process(g.Node, g.Elem)
     K = O.model_K_csr(g.Node, g.Elem, pp, g.EX, g.PO, g.Bounds)
-    n, nb = K.shape[0], 8
+    n = K.shape[0]
+    assert model.dev.grid_nodes is not None
     Bh = np.random.default_rng(9).standard_normal((n, nb))
     Bh[:, 3] = 0.0
     X, info = MG.solve_block_fcg_mg(model, K64, torch.as_tensor(Bh).cuda(), pre, tol=1e-9)
@@ -285,6 +289,83 @@ def test_fcg_steps_match_numpy(gpu, n, nb):
     again = run()
     for a, b in zip(again, (P, X, R, rr, R32)):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,nb,nblk", [(7, 1, 1), (77777, 1, 13), (30000, 8, 300), (20000, 16, 2016), (300, 64, 5)])
+def test_fcg_split_steps_match_numpy(gpu, n, nb, nblk):
+    """fem_fcg_direction_pr / fem_fcg_update_split (p.q from nblk outside
+    partials, p.r from the direction pass) against the same two FCG(1)
+    iterations in numpy as test_fcg_steps_match_numpy, and against the
+    unsplit steps; the tolerance is test_fcg_steps_match_numpy's (random P
+    and Q make some p.q small, and the partials here sum in another order)."""
+    import torch
+    from fem_taichi_amd import multigrid as MG
+    print(f"\nfcg split steps n {n} nb {nb} partials {nblk}", flush=True)
+    rng = np.random.default_rng(5 * n + nb)
+    mask = (rng.random(n) > 0.1).astype(np.float64)
+    active = np.ones(nb, dtype=np.int32)
+    active[nb // 2] = 0 if nb > 1 else 1
+    X0, R0 = rng.standard_normal((n, nb)), rng.standard_normal((n, nb)) * mask[:, None]
+    Z1, Z2 = rng.standard_normal((n, nb)).astype(np.float32), rng.standard_normal((n, nb))
+    Q1, Q2 = rng.standard_normal((n, nb)), rng.standard_normal((n, nb))
+    c = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+
+    def partials(P, Q):
+        # nblk row ranges of unequal length (some empty when nblk > n)
+        cuts = np.sort(rng.integers(0, n + 1, nblk - 1))
+        edges = np.concatenate([[0], cuts, [n]])
+        return np.stack([(P[a:b] * Q[a:b]).sum(0) for a, b in zip(edges[:-1], edges[1:])])
+
+    def run(split):
+        st = MG.FcgSteps(n, nb, "cuda")
+        X, R, P = c(X0), c(R0), torch.empty(n, nb, dtype=torch.float64, device="cuda")
+        R32 = torch.empty(n, nb, dtype=torch.float32, device="cuda")
+        m, act, q1, q2 = c(mask), c(active), c(Q1), c(Q2)
+        out = []
+        for Z, Q, Qp, first in ((Z1, q1, None, True), (Z2, q2, q1, False)):
+            if split:
+                st.direction_pr(c(Z), m, Qp, R, P, first)
+                pq = c(partials(P.cpu().numpy(), Q.cpu().numpy()))
+                rr = st.update_split(P, Q, m, act, pq, nblk, X, R, R32)
+            else:
+                st.direction(c(Z), m, Qp, P, first)
+                rr = st.update(P, Q, m, act, X, R, R32)
+        return P.cpu().numpy(), X.cpu().numpy(), R.cpu().numpy(), rr.cpu().numpy(), R32.cpu().numpy()
+
+    P, X, R, rr, R32 = run(True)
+    P0, X0u, R0u, rr0, _ = run(False)
+    for a, b in ((P, P0), (X, X0u), (R, R0u)):
+        assert rel(a, b) < 1e-10
+    assert np.allclose(rr, (R * R).sum(0), rtol=1e-12, atol=0)
+    assert np.array_equal(R32, R.astype(np.float32))
+    if nb > 1:
+        assert np.array_equal(X[:, nb // 2], X0[:, nb // 2])
+    # the formulas themselves (as in test_fcg_steps_match_numpy)
+    Xr, Rr = X0.copy(), R0.copy()
+    Pr = mask[:, None] * Z1.astype(np.float64)
+    pq = (Pr * Q1).sum(0)
+    alpha = np.where(active > 0, (Pr * Rr).sum(0) / pq, 0.0)
+    Xr += alpha * Pr
+    Rr -= alpha * mask[:, None] * Q1
+    zm = mask[:, None] * Z2
+    Pr = zm - ((zm * Q1).sum(0) / pq) * Pr
+    alpha2 = np.where(active > 0, (Pr * Rr).sum(0) / (Pr * Q2).sum(0), 0.0)
+    Xr += alpha2 * Pr
+    Rr -= alpha2 * mask[:, None] * Q2
+    for a, b in ((P, Pr), (X, Xr), (R, Rr)):
+        assert rel(a, b) < 1e-10
+
+
+def test_fcg_split_rejects_bad_partials(gpu):
+    import torch
+    from fem_taichi_amd import _lib, multigrid as MG
+    st = MG.FcgSteps(10, 2, "cuda")
+    V = torch.zeros(10, 2, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        st.update_split(V, V, None, None, torch.zeros(3, dtype=torch.float64, device="cuda"), 2, V, V.clone(), None)
+    with pytest.raises(_lib.FemError):
+        _lib.call("fem_fcg_update_split", 10, 2, V.data_ptr(), V.data_ptr(), None, None, V.data_ptr(), 0,
+                  V.data_ptr(), V.data_ptr(), None, st.rr.data_ptr(), st.work.data_ptr(), st.wb, None)
 
 
 @pytest.mark.parametrize("n,nb", [(5, 1), (77777, 1), (30000, 8), (20000, 16), (300, 64)])

@@ -126,3 +126,44 @@ def test_mass16_grid_matches_scipy(gpu, dims):
     Y2 = torch.empty_like(Xd)
     _lib.call("fem_mass_spmm", phys.dev.ref, mb.data_ptr(), Xd.data_ptr(), Y2.data_ptr(), 16, stream_ptr())
     assert np.linalg.norm(Y.cpu().numpy() - Y2.cpu().numpy()) <= 1e-13 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("dims", GRIDS)
+def test_spmm16_grid_pq_matches_product(gpu, dims):
+    """fem_bsr_spmm16_grid_pq: Y bit-identical to fem_bsr_spmm16_grid's, and
+    the per-work-group partials summing, per column, to X_c . Y_c within
+    1e-13 of numpy's; repeat calls bit-identical; a partial buffer shorter
+    than 16 x fem_spmm16_grid_partials is refused."""
+    import torch
+    from fem_taichi_amd import _lib
+    from fem_taichi_amd.runtime import stream_ptr
+    print(f"\nspmm16 grid pq {dims}", flush=True)
+    g, phys, model = build(dims, seed=13)
+    N = phys.Num_Node
+    k = model.assemble_K("f64", "bsr", penalty=False)
+    rng = np.random.default_rng(17 + sum(dims))
+    Xd = torch.as_tensor(rng.standard_normal((3 * N, 16))).cuda()
+    nx, ny, nz = phys.dev.grid_nodes
+    nblk = int(_lib.lib().fem_spmm16_grid_partials(nx, ny, nz))
+    assert nblk >= 1
+    part = torch.full((nblk * 16,), np.nan, dtype=torch.float64, device="cuda")
+    Y = torch.full_like(Xd, np.nan)
+    _lib.call("fem_bsr_spmm16_grid_pq", phys.dev.ref, nx, ny, nz, k.data_ptr(), Xd.data_ptr(), Y.data_ptr(),
+              part.data_ptr(), part.numel(), stream_ptr())
+    Y0 = torch.full_like(Xd, np.nan)
+    _lib.call("fem_bsr_spmm16_grid", phys.dev.ref, nx, ny, nz, k.data_ptr(), Xd.data_ptr(), Y0.data_ptr(),
+              stream_ptr())
+    assert torch.equal(Y, Y0)
+    ph = part.cpu().numpy().reshape(nblk, 16)
+    assert np.all(np.isfinite(ph))
+    X, Yh = Xd.cpu().numpy(), Y.cpu().numpy()
+    ref = (X * Yh).sum(0)
+    scale = (np.abs(X) * np.abs(Yh)).sum(0)
+    assert np.all(np.abs(ph.sum(0) - ref) <= 1e-13 * scale)
+    part2 = torch.full_like(part, np.nan)
+    _lib.call("fem_bsr_spmm16_grid_pq", phys.dev.ref, nx, ny, nz, k.data_ptr(), Xd.data_ptr(), Y.data_ptr(),
+              part2.data_ptr(), part2.numel(), stream_ptr())
+    assert torch.equal(part, part2)
+    with pytest.raises(_lib.FemError):
+        _lib.call("fem_bsr_spmm16_grid_pq", phys.dev.ref, nx, ny, nz, k.data_ptr(), Xd.data_ptr(), Y.data_ptr(),
+                  part2.data_ptr(), part2.numel() - 1, stream_ptr())
