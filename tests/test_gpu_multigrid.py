@@ -41,16 +41,27 @@ def test_transfers_match_interpolation_matrix(gpu, dims):
     y = torch.as_tensor(xf).cuda()
     _lib.call("fem_mg_prolong_add", T.ref, torch.as_tensor(xc).cuda().data_ptr(), y.data_ptr(), 1, stream_ptr())
     assert rel(y.cpu().numpy(), xf + P @ xc.astype(np.float64)) < 1e-6
-    # several column-interleaved vectors at once
-    nb = 5
-    XF = rng.standard_normal((P.shape[0], nb)).astype(np.float32)
-    XC = rng.standard_normal((P.shape[1], nb)).astype(np.float32)
-    BC = torch.empty(P.shape[1], nb, dtype=torch.float32, device="cuda")
-    _lib.call("fem_mg_restrict", T.ref, torch.as_tensor(XF).cuda().data_ptr(), BC.data_ptr(), nb, stream_ptr())
-    assert rel(BC.cpu().numpy(), P.T @ XF.astype(np.float64)) < 1e-6
-    Y = torch.as_tensor(XF).cuda()
-    _lib.call("fem_mg_prolong_add", T.ref, torch.as_tensor(XC).cuda().data_ptr(), Y.data_ptr(), nb, stream_ptr())
-    assert rel(Y.cpu().numpy(), XF + P @ XC.astype(np.float64)) < 1e-6
+    # several column-interleaved vectors at once (5: one column per thread;
+    # 8 and 16: four per thread, each column's sums in the same order, so
+    # bit-identical to single-vector calls on the columns)
+    for nb in (5, 8, 16):
+        XF = rng.standard_normal((P.shape[0], nb)).astype(np.float32)
+        XC = rng.standard_normal((P.shape[1], nb)).astype(np.float32)
+        BC = torch.empty(P.shape[1], nb, dtype=torch.float32, device="cuda")
+        _lib.call("fem_mg_restrict", T.ref, torch.as_tensor(XF).cuda().data_ptr(), BC.data_ptr(), nb, stream_ptr())
+        assert rel(BC.cpu().numpy(), P.T @ XF.astype(np.float64)) < 1e-6
+        Y = torch.as_tensor(XF).cuda()
+        _lib.call("fem_mg_prolong_add", T.ref, torch.as_tensor(XC).cuda().data_ptr(), Y.data_ptr(), nb, stream_ptr())
+        assert rel(Y.cpu().numpy(), XF + P @ XC.astype(np.float64)) < 1e-6
+        for c in range(nb):
+            b1 = torch.empty(P.shape[1], dtype=torch.float32, device="cuda")
+            _lib.call("fem_mg_restrict", T.ref, torch.as_tensor(np.ascontiguousarray(XF[:, c])).cuda().data_ptr(),
+                      b1.data_ptr(), 1, stream_ptr())
+            assert torch.equal(b1, BC[:, c]), (nb, c)
+            y1 = torch.as_tensor(np.ascontiguousarray(XF[:, c])).cuda()
+            _lib.call("fem_mg_prolong_add", T.ref, torch.as_tensor(np.ascontiguousarray(XC[:, c])).cuda().data_ptr(),
+                      y1.data_ptr(), 1, stream_ptr())
+            assert torch.equal(y1, Y[:, c]), (nb, c)
 
 
 @pytest.mark.parametrize("layout", ["bsr", "csr"])
