@@ -205,6 +205,18 @@ def hierarchy_for(model, min_nodes=1000):
     return cache[key]
 
 
+_SIDE_STREAMS = {}
+
+
+def _side_stream(dev):
+    """One second stream per device (default priority) for one-time setup
+    work that can run beside the main stream's."""
+    key = str(dev)
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = torch.cuda.Stream(device=dev)
+    return _SIDE_STREAMS[key]
+
+
 class MultigridPreconditioner:
     """FP32 V-cycle z = MG(r) for FP64 r (numeric setup in the constructor)."""
 
@@ -218,6 +230,7 @@ class MultigridPreconditioner:
         self.meshes, self.k32, self.dinv, self.omega = [], [], [], []
         EX = model.EX
         mats = {"EX": EX, "PO": model.PO, "DE": model.DE}
+        models = []
         for l in range(L):
             if l == 0:
                 m = model
@@ -225,16 +238,30 @@ class MultigridPreconditioner:
                 mats = {"EX": coarse_modulus(mats["EX"], hier.dims[l - 1], hier.sels[l - 1]), "PO": model.PO,
                         "DE": model.DE}
                 m = Model(hier.phys[l].name, hier.phys[l], mats, Omegas=(0.0, 0.0, 0.0))
+            models.append(m)
             self.meshes.append(m.dev)
-            if l < L - 1:
-                k32 = m.K_bsr("f32") if l == 0 else m.assemble_K("f32", "bsr")
-                dinv, _ = BsrOperator(m.dev, k32).block_jacobi()
-                self.k32.append(k32)
-                self.dinv.append(dinv)
-                self.omega.append(1.0 / self._theta(m.dev, k32, dinv, seed + l))
-            else:
-                self.k_coarse = m.assemble_K("f64", "bsr")
-                self.coarse_inv = self._dense_inverse(m, self.k_coarse)
+        # The coarsest level's dense inverse (rocSOLVER's Cholesky: many small,
+        # latency-bound kernels) runs on a second stream beside the finer
+        # levels' assembly and power steps (bandwidth-bound); both are one-time
+        # numeric setup.  Its status is read only after the join.
+        main = torch.cuda.current_stream()
+        self.k_coarse = models[-1].assemble_K("f64", "bsr")
+        side = _side_stream(d)
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            coarse_inv, info = self._dense_inverse(models[-1], self.k_coarse)
+        for l in range(L - 1):
+            m = models[l]
+            k32 = m.K_bsr("f32") if l == 0 else m.assemble_K("f32", "bsr")
+            dinv, _ = BsrOperator(m.dev, k32).block_jacobi()
+            self.k32.append(k32)
+            self.dinv.append(dinv)
+            self.omega.append(1.0 / self._theta(m.dev, k32, dinv, seed + l))
+        main.wait_stream(side)
+        coarse_inv.record_stream(main)
+        if int(info) != 0:
+            raise _lib.FemError(_lib.FEM_E_NUMERIC, "multigrid: the coarsest K is not positive definite")
+        self.coarse_inv = coarse_inv
         n = [3 * int(np.prod(dm)) for dm in hier.dims]
         self.b = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
         self.x = [torch.empty(k, dtype=torch.float32, device=d) for k in n]
@@ -280,7 +307,9 @@ class MultigridPreconditioner:
 
     @staticmethod
     def _dense_inverse(m, vals64):
-        """FP32 copy of the inverse of the coarsest (penalised) K, by FP64 Cholesky."""
+        """FP32 copy of the inverse of the coarsest (penalised) K, by FP64
+        Cholesky, and the factorisation's status (device; nonzero: not
+        positive definite), queued on the current stream without waiting."""
         N = m.Num_Node
         d = device()
         co = m.dev["co_node"].long()
@@ -291,10 +320,8 @@ class MultigridPreconditioner:
         A = torch.zeros(3 * N, 3 * N, dtype=torch.float64, device=d)
         A[r3.reshape(-1), c3.reshape(-1)] = vals64.view(-1)
         Lc, info = torch.linalg.cholesky_ex(A)
-        if int(info) != 0:
-            raise _lib.FemError(_lib.FEM_E_NUMERIC, "multigrid: the coarsest K is not positive definite")
         inv = torch.cholesky_inverse(Lc)
-        return inv.float().contiguous()
+        return inv.float().contiguous(), info
 
     def vcycle(self, b0: torch.Tensor) -> torch.Tensor:
         """x = MG(b0) in FP32 (b0 FP32 on the finest level)."""
