@@ -223,7 +223,8 @@ def test_mg_static_solver_method(gpu):
 @pytest.mark.parametrize("nb", [1, 3, 8, 13, 16, 64])
 def test_coarsest_dense_products(gpu, n, nb):
     """fem_dense_gemm_nb_f32 (the coarsest level's inverse on nb columns, in
-    16-row x 16-column tiles) and fem_dense_gemv_f32 against numpy in FP64."""
+    4-row x 16-column tiles, k split over four slices) and fem_dense_gemv_f32
+    against numpy in FP64."""
     import torch
     from fem_taichi_amd import _lib
     from fem_taichi_amd.runtime import stream_ptr
